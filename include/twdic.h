@@ -1,0 +1,193 @@
+/*
+ * twdic.h -- C ABI of the MI355X-native paDIC hot path (FFT-CC integer-pixel
+ * search + first-order IC-GN with bicubic interpolation).
+ *
+ * Plain pointers and sizes only.  Every call returns a twdic_status and never
+ * exits the process (the reference calls exit() on any CUDA error,
+ * helper_cuda.h:964-997).  One context owns one HIP stream and all device
+ * state; calls on one context are asynchronous on that stream unless stated.
+ * A context is not thread-safe; use one per host thread.
+ *
+ * Pointer convention: arguments named d_* are device pointers (allocate them
+ * with twdic_malloc or any HIP allocator on the context's device); arguments
+ * named h_* are host pointers.  Images are 8-bit single channel, row-major,
+ * with a row stride in bytes.
+ *
+ * The entries below replace the reference's class methods; each comment
+ * names the reference interface it replaces (paths relative to the reference
+ * checkout).
+ */
+#ifndef TWDIC_H
+#define TWDIC_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct twdic_ctx twdic_ctx;
+
+typedef enum {
+    TWDIC_OK = 0,
+    TWDIC_EINVAL = 1,   /* bad argument or geometry */
+    TWDIC_ENOMEM = 2,   /* device or pinned allocation failed */
+    TWDIC_EBOUNDS = 3,  /* a POI window/subset leaves the image or ROI */
+    TWDIC_EHIP = 4,     /* HIP runtime error (message in twdic_last_error) */
+    TWDIC_EBUSY = 5,    /* serving queue full (admission refused) */
+    TWDIC_ENODEV = 6    /* no usable GPU */
+} twdic_status;
+
+/* Per-POI IC-GN result flag: ICGN2DFlag (TW/TW_Engine/TW_paDIC.h:74-81). */
+enum { TWDIC_ICGN_SUCCESS = 0, TWDIC_ICGN_DARK_SUBSET = 1,
+       TWDIC_ICGN_SINGULAR_HESSIAN = 2, TWDIC_ICGN_SINGULAR_WARP = 3,
+       TWDIC_ICGN_OUT_OF_ROI = 4 };
+
+/* Per-POI FFT-CC status.  FLAT: a window is uniform; the reference returns
+ * early and leaves u/v/zncc untouched (TW_paDIC_FFTCC2D_CPU.cpp:271-275), and
+ * so does this library.  OUT: the N x N window leaves the image (outputs
+ * untouched; the reference reads out of bounds there). */
+enum { TWDIC_FFTCC_OK = 0, TWDIC_FFTCC_FLAT = 1, TWDIC_FFTCC_OUT = 2 };
+
+/* IC-GN interpolation source. */
+enum {
+    TWDIC_INTERP_LUT = 0,   /* materialise the 16-coefficient bicubic LUT of the
+                               target each frame and gather from it, as the
+                               reference does (TW_cuUtils.cu:314-435) */
+    TWDIC_INTERP_FUSED = 1  /* evaluate the same bicubic interpolant directly
+                               from the target pixels (tensor Catmull-Rom, which
+                               equals the LUT polynomial built from central-
+                               difference derivatives); no LUT traffic */
+};
+
+/* Configuration: the union of the reference constructors' arguments
+ * (cuFFTCC2D, TW_paDIC_cuFFTCC2D.h:52-57; cuICGN2D, TW_paDIC_cuICGN2D.h:14-22).
+ * fft_n: FFT-CC window edge N (even, 2..64); N = 2*sx reproduces the
+ * reference (TW_paDIC_FFTCC2D_CPU.cpp:24-25). */
+typedef struct {
+    int img_w, img_h;
+    int roi_x, roi_y, roi_w, roi_h;
+    int sx, sy;          /* subset half sizes: subset (2sy+1) x (2sx+1) */
+    int gx, gy;          /* POI grid spacing */
+    int mx, my;          /* ROI margins */
+    int fft_n;           /* FFT-CC window edge */
+    int max_iter;        /* IC-GN maximum iterations (reference: 20) */
+    float delta_p;       /* IC-GN convergence threshold (reference: 0.001) */
+    int interp;          /* TWDIC_INTERP_* */
+} twdic_config;
+
+/* ---------------------------------------------------------------- context */
+
+/* Replaces the cuFFTCC2D / cuICGN2D constructors + cuICGN2D::prepare
+ * (TW_paDIC_cuFFTCC2D.cu:192-210, TW_paDIC_cuICGN2D.cu:694-768):
+ * validates the geometry (std::logic_error "Number of POIs is below 0!" in
+ * the reference becomes TWDIC_EINVAL) and allocates ROI-sized state. */
+twdic_status twdic_create(const twdic_config* cfg, int device, twdic_ctx** out);
+/* Replaces ~cuFFTCC2D / ~cuICGN2D / cuDestroyFFTCC / cuFinalize
+ * (TW_paDIC_cuFFTCC2D.cu:212-233,649-677; TW_paDIC_cuICGN2D.cu:719-729,970-989).
+ * Frees only what the context allocated (borrowed pointers are never freed). */
+twdic_status twdic_destroy(twdic_ctx* ctx);
+const char* twdic_last_error(const twdic_ctx* ctx);
+/* Blocks until all work queued on the context's stream has finished. */
+twdic_status twdic_sync(twdic_ctx* ctx);
+/* The context's hipStream_t, as void*. */
+void* twdic_stream(twdic_ctx* ctx);
+
+/* POI counts, Fftcc2D::GetNumPOIsX/Y (TW_paDIC_FFTCC2D.h:99-101) after
+ * recomputeNumPOI (TW_paDIC_FFTCC2D.cpp:88-94). */
+twdic_status twdic_num_poi(const twdic_config* cfg, int32_t* nx, int32_t* ny);
+
+/* ------------------------------------------------------------------ images */
+
+/* Replaces cuFFTCC2D::ResetRefImg / cuICGN2D::cuInitialize / ResetRefImg
+ * (TW_paDIC_cuFFTCC2D.cu:679-685; TW_paDIC_cuICGN2D.cu:770-793).  Copies the
+ * image into the context (h_img host, or d_img device when on_device != 0;
+ * stride in bytes) and marks the per-reference state stale; it is rebuilt
+ * lazily by the next IC-GN call for the POI set used there. */
+twdic_status twdic_set_reference(twdic_ctx* ctx, const uint8_t* img, int stride, int on_device);
+/* Replaces cuFFTCC2D::SetTarImg / the target upload in cuComputeFFTCC
+ * (TW_paDIC_cuFFTCC2D.cu:560-563,687-693).  Host images are staged through
+ * pinned memory and copied asynchronously on the context's stream. */
+twdic_status twdic_set_target(twdic_ctx* ctx, const uint8_t* img, int stride, int on_device);
+
+/* -------------------------------------------------------------- POI grid */
+
+/* Replaces cuComputePOIPositions (TW_cuUtils.cu:496-605) and
+ * ComputePOIPositions_s/_m (TW_utils.cpp:8-94): writes int32[n][2] = {y, x}
+ * (y first, whole-image coordinates) to d_poi_yx.  n = nx*ny. */
+twdic_status twdic_poi_grid(twdic_ctx* ctx, int32_t* d_poi_yx);
+
+/* ---------------------------------------------------------------- FFT-CC */
+
+/* Replaces cuFFTCC2D::cuComputeFFTCC's device pipeline (cufft_prepare_kernel,
+ * cufftExecR2C x2, complexMulandScale_kernel, cufftExecC2R, findMax;
+ * TW_paDIC_cuFFTCC2D.cu:575-638) with CPU-path semantics
+ * (Fftcc2D_CPU::ComputeFFTCC, TW_paDIC_FFTCC2D_CPU.cpp:222-326):
+ * integer peak = exact argmax of the circular cross-correlation of the two
+ * N x N windows, lowest row-major index on ties, wrapped to (-N/2, N/2];
+ * ZNCC = peak / (N^2 sqrt(sum R~^2 sum T~^2)).  d_status may be NULL. */
+twdic_status twdic_fftcc(twdic_ctx* ctx, const int32_t* d_poi_yx, int n,
+                         float* d_u, float* d_v, float* d_zncc, uint8_t* d_status);
+
+/* ----------------------------------------------------------------- IC-GN */
+
+/* Replaces cuICGN2D::cuCompute (TW_paDIC_cuICGN2D.cu:811-968) with CPU-path
+ * semantics (ICGN2D_CPU::ICGN2D_Compute, TW_paDIC_ICGN2D_CPU.cpp:481-925):
+ * per-frame target precompute (gradients + bicubic LUT when interp = LUT),
+ * per-reference state (reference subsets, steepest-descent sums, FP64
+ * Cholesky factor of the Hessian) rebuilt when the reference or the POI set
+ * changed, then the IC-GN iterations.
+ * d_u/d_v: in = initial guess (e.g. FFT-CC result), out = sub-pixel result.
+ * d_p6 (may be NULL): final (u, ux, uy, v, vx, vy) per POI.
+ * d_iters (may be NULL): 1-based iteration count; untouched when a flag is
+ * raised before the first update (as in the reference).
+ * d_flags (may be NULL): ICGN2DFlag per POI.
+ * d_dpnorm (may be NULL): weighted |dp| of the last update (audit aid). */
+twdic_status twdic_icgn(twdic_ctx* ctx, const int32_t* d_poi_yx, int n,
+                        float* d_u, float* d_v, float* d_p6, int32_t* d_iters,
+                        uint8_t* d_flags, float* d_dpnorm);
+
+/* The plain per-frame step of FFTCCTWorkerThread::processFrameFFTCC_ICGN
+ * (TW/TW_Core/fftcctworkerthread.cpp:445-461): FFT-CC then IC-GN on the
+ * current target, results left on the device. */
+twdic_status twdic_frame(twdic_ctx* ctx, const int32_t* d_poi_yx, int n,
+                         float* d_u, float* d_v, float* d_zncc, float* d_p6,
+                         int32_t* d_iters, uint8_t* d_flags);
+
+/* ------------------------------------------------ precompute (inspection) */
+
+/* Device buffers the reference exposes as public members of
+ * GPUHandle_ICGN (TW/TW_Engine/TW_paDIC.h:41-70): gradients of the reference
+ * (Rx, Ry) and target (Tx, Ty, Txy), ROI-relative row-major fp32; the target
+ * bicubic LUT, [roi_h][roi_w][16] fp32 with element 4k+l multiplying y^k x^l
+ * (same order as the CPU LUT, TW_utils.cpp:673-688; the last ROI row and
+ * column are zero).  Target buffers are (re)computed by twdic_precompute_target. */
+enum { TWDIC_BUF_RX = 0, TWDIC_BUF_RY = 1, TWDIC_BUF_TX = 2, TWDIC_BUF_TY = 3,
+       TWDIC_BUF_TXY = 4, TWDIC_BUF_LUT = 5, TWDIC_BUF_REF = 6, TWDIC_BUF_TAR = 7 };
+twdic_status twdic_get_buffer(twdic_ctx* ctx, int which, void** d_ptr, size_t* bytes);
+/* cuGradientXY_2Images + cuBicubicCoefficients (TW_cuUtils.cu:679-776):
+ * gradients of both images and the target LUT, written to the buffers above. */
+twdic_status twdic_precompute_target(twdic_ctx* ctx);
+
+/* ---------------------------------------------------------- device memory */
+
+twdic_status twdic_malloc(twdic_ctx* ctx, void** d_ptr, size_t bytes);
+twdic_status twdic_free(twdic_ctx* ctx, void* d_ptr);
+/* Asynchronous on the context stream when h_ptr is pinned; synchronous
+ * otherwise.  kind: 0 = host to device, 1 = device to host, 2 = device to
+ * device. */
+twdic_status twdic_memcpy(twdic_ctx* ctx, void* dst, const void* src, size_t bytes, int kind);
+twdic_status twdic_memset(twdic_ctx* ctx, void* d_ptr, int value, size_t bytes);
+
+/* ------------------------------------------------------------- timing */
+
+/* Events recorded on the context stream, for device-side timing. */
+twdic_status twdic_event_record(twdic_ctx* ctx, int slot);
+/* Milliseconds between two recorded slots (synchronises on the later one). */
+twdic_status twdic_event_elapsed(twdic_ctx* ctx, int slot_a, int slot_b, float* ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

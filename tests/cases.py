@@ -1,0 +1,65 @@
+"""Workload definitions shared by tests and the benchmark (sizes, seeds, POI
+grids) -- the configurations of the speckle cases named in each helper."""
+from __future__ import annotations
+
+from dataclasses import dataclass
+
+import numpy as np
+
+from tw_amd import api
+
+
+@dataclass
+class Case:
+    name: str
+    img: int
+    S: int
+    grid: int
+    N: int
+    roi: tuple            # (start_x, start_y, w, h)
+    margin: int
+    ref: np.ndarray
+    tar: np.ndarray
+    deform: api.Deform | None
+
+    def config(self, interp=api.INTERP_FUSED, N=None, max_iter=20, delta_p=0.001) -> api.Config:
+        x, y, w, h = self.roi
+        return api.make_config(self.img, self.img, x, y, w, h, self.S, self.S, self.grid, self.grid,
+                               self.margin, self.margin, N or self.N, max_iter, delta_p, interp)
+
+    def poi(self) -> np.ndarray:
+        import math
+        x, y, w, h = self.roi
+        nx = int(math.floor((w - 2 * self.S - 2 * self.margin) / float(self.grid))) + 1
+        ny = int(math.floor((h - 2 * self.S - 2 * self.margin) / float(self.grid))) + 1
+        iy, ix = np.meshgrid(np.arange(ny), np.arange(nx), indexing="ij")
+        py = y + self.margin + self.S + iy * self.grid
+        px = x + self.margin + self.S + ix * self.grid
+        return np.stack([py.ravel(), px.ravel()], 1).astype(np.int32)
+
+
+def speckle_512(seed=1001) -> Case:
+    """512x512, 4000 spots, target = Catmull-Rom resample under u=2.35, v=-1.72,
+    noise sigma 2; POIs on a 10 px grid with a 20 px border, 31x31 subsets,
+    FFT-CC window 32."""
+    ref, tar = api.synth_pair(512, 512, 4000, seed, api.resample(2.35, -1.72), noise_sigma=2.0)
+    # POI = start + margin + S = 1 + 4 + 15 = 20
+    return Case("speckle512_S15_g10", 512, 15, 10, 32, (1, 1, 510, 510), 4, ref, tar, api.resample(2.35, -1.72))
+
+
+def speckle_2048(seed=1002) -> Case:
+    """2048x2048, 64000 spots, affine exx=0.5%, eyy=-0.2%, exy=0.1%, u=5.6,
+    v=-3.3 about the centre, noise sigma 2; POIs on a 5 px grid from a 34 px
+    border (max(N/2, S + ceil(max|u|) + 2)), 41x41 subsets, FFT-CC window 64."""
+    d = api.affine(5.6, -3.3, 0.005, -0.002, 0.001, 1023.5, 1023.5)
+    ref, tar = api.synth_pair(2048, 2048, 64000, seed, d, noise_sigma=2.0)
+    # POI = 1 + 13 + 20 = 34; 397 x 397 = 157,609 POIs
+    return Case("speckle2048_S20_g5", 2048, 20, 5, 64, (1, 1, 2046, 2046), 13, ref, tar, d)
+
+
+def truth(case: Case, poi: np.ndarray) -> np.ndarray:
+    """Analytic (u, ux, uy, v, vx, vy) at each POI of an affine case."""
+    out = np.zeros((len(poi), 6))
+    for i, (y, x) in enumerate(poi):
+        out[i] = api.displacement(case.deform, float(x), float(y))
+    return out

@@ -1,0 +1,286 @@
+"""ctypes binding of the C ABI in include/twdic.h (libtwdic.so, built in-tree).
+
+This is plumbing for tests and benchmarks: every call goes straight to the
+HIP library.  There is no CPU fallback; if the library is missing or no GPU is
+usable, the calls raise.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+import numpy as np
+
+LIB_PATH = Path(__file__).resolve().parent / "lib" / "libtwdic.so"
+
+# status codes (twdic_status)
+OK, EINVAL, ENOMEM, EBOUNDS, EHIP, EBUSY, ENODEV = range(7)
+# ICGN2DFlag
+ICGN_SUCCESS, ICGN_DARK_SUBSET, ICGN_SINGULAR_HESSIAN, ICGN_SINGULAR_WARP, ICGN_OUT_OF_ROI = range(5)
+FFTCC_OK, FFTCC_FLAT, FFTCC_OUT = range(3)
+INTERP_LUT, INTERP_FUSED = 0, 1
+BUF_RX, BUF_RY, BUF_TX, BUF_TY, BUF_TXY, BUF_LUT, BUF_REF, BUF_TAR = range(8)
+DEFORM_AFFINE, DEFORM_SINUSOID, DEFORM_RESAMPLE = range(3)
+
+
+class TwdicError(RuntimeError):
+    pass
+
+
+class Config(C.Structure):
+    _fields_ = [(n, C.c_int) for n in ("img_w", "img_h", "roi_x", "roi_y", "roi_w", "roi_h", "sx", "sy",
+                                       "gx", "gy", "mx", "my", "fft_n", "max_iter")] + [
+        ("delta_p", C.c_float), ("interp", C.c_int)]
+
+
+class Deform(C.Structure):
+    _fields_ = [("kind", C.c_int)] + [(n, C.c_double) for n in (
+        "u0", "v0", "exx", "eyy", "exy", "xc", "yc", "amp_u", "per_u", "amp_v", "per_v")]
+
+
+class Speckle(C.Structure):
+    _fields_ = [("w", C.c_int), ("h", C.c_int), ("nspots", C.c_int), ("seed", C.c_uint64),
+                ("sigma_min", C.c_double), ("sigma_max", C.c_double), ("amp_min", C.c_double),
+                ("amp_max", C.c_double), ("noise_sigma", C.c_double), ("noise_seed", C.c_uint64),
+                ("boxes", C.c_void_p), ("nboxes", C.c_int)]
+
+
+_lib = None
+
+_VP, _I, _SZ = C.c_void_p, C.c_int, C.c_size_t
+_SIGS = {
+    "twdic_create": ([C.POINTER(Config), _I, C.POINTER(_VP)], _I),
+    "twdic_destroy": ([_VP], _I),
+    "twdic_last_error": ([_VP], C.c_char_p),
+    "twdic_sync": ([_VP], _I),
+    "twdic_stream": ([_VP], _VP),
+    "twdic_num_poi": ([C.POINTER(Config), C.POINTER(C.c_int32), C.POINTER(C.c_int32)], _I),
+    "twdic_set_reference": ([_VP, _VP, _I, _I], _I),
+    "twdic_set_target": ([_VP, _VP, _I, _I], _I),
+    "twdic_poi_grid": ([_VP, _VP], _I),
+    "twdic_fftcc": ([_VP, _VP, _I, _VP, _VP, _VP, _VP], _I),
+    "twdic_icgn": ([_VP, _VP, _I, _VP, _VP, _VP, _VP, _VP, _VP], _I),
+    "twdic_frame": ([_VP, _VP, _I, _VP, _VP, _VP, _VP, _VP, _VP], _I),
+    "twdic_get_buffer": ([_VP, _I, C.POINTER(_VP), C.POINTER(_SZ)], _I),
+    "twdic_precompute_target": ([_VP], _I),
+    "twdic_malloc": ([_VP, C.POINTER(_VP), _SZ], _I),
+    "twdic_free": ([_VP, _VP], _I),
+    "twdic_memcpy": ([_VP, _VP, _VP, _SZ, _I], _I),
+    "twdic_memset": ([_VP, _VP, _I, _SZ], _I),
+    "twdic_event_record": ([_VP, _I], _I),
+    "twdic_event_elapsed": ([_VP, _I, _I, C.POINTER(C.c_float)], _I),
+    "twdic_synth_pair": ([C.POINTER(Speckle), C.POINTER(Deform), _VP, _VP, _I], _I),
+    "twdic_synth_displacement": ([C.POINTER(Deform), C.c_double, C.c_double, C.POINTER(C.c_double)], None),
+}
+
+
+def lib() -> C.CDLL:
+    """Load libtwdic.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise TwdicError(f"{LIB_PATH} is missing: run __graft_entry__.build() or python tw_amd/build.py")
+        L = C.CDLL(str(LIB_PATH))
+        for name, (args, res) in _SIGS.items():
+            f = getattr(L, name)
+            f.argtypes = args
+            f.restype = res
+        _lib = L
+    return _lib
+
+
+def exported_symbols() -> list[str]:
+    return list(_SIGS)
+
+
+def _ptr(a: np.ndarray) -> int:
+    return a.ctypes.data
+
+
+# --------------------------------------------------------------------------- synth
+def synth_pair(w: int, h: int, nspots: int, seed: int, deform: Deform | None = None, *,
+               sigma=(2.0, 4.0), amp=(60.0, 200.0), noise_sigma=2.0, noise_seed: int | None = None,
+               boxes: np.ndarray | None = None, want_ref=True, want_tar=True):
+    """Deterministic speckle pair (reference, deformed target) as uint8 arrays."""
+    sp = Speckle(w, h, nspots, seed, sigma[0], sigma[1], amp[0], amp[1], noise_sigma,
+                 (seed ^ 0xA5A5A5A5) if noise_seed is None else noise_seed, None, 0)
+    bx = None
+    if boxes is not None and len(boxes):
+        bx = np.ascontiguousarray(boxes, dtype=np.int32)
+        sp.boxes = _ptr(bx)
+        sp.nboxes = len(bx)
+    ref = np.zeros((h, w), np.uint8) if want_ref else None
+    tar = np.zeros((h, w), np.uint8) if want_tar else None
+    d = deform if deform is not None else Deform(DEFORM_AFFINE, 0, 0, 0, 0, 0, w / 2, h / 2, 0, 1, 0, 1)
+    rc = lib().twdic_synth_pair(C.byref(sp), C.byref(d), _ptr(ref) if ref is not None else None,
+                                _ptr(tar) if tar is not None else None, w)
+    if rc != 0:
+        raise TwdicError("twdic_synth_pair failed")
+    return ref, tar
+
+
+def affine(u0, v0, exx=0.0, eyy=0.0, exy=0.0, xc=0.0, yc=0.0) -> Deform:
+    return Deform(DEFORM_AFFINE, u0, v0, exx, eyy, exy, xc, yc, 0.0, 1.0, 0.0, 1.0)
+
+
+def sinusoid(amp_u, per_u, amp_v, per_v) -> Deform:
+    return Deform(DEFORM_SINUSOID, 0, 0, 0, 0, 0, 0, 0, amp_u, per_u, amp_v, per_v)
+
+
+def resample(u0, v0) -> Deform:
+    return Deform(DEFORM_RESAMPLE, u0, v0, 0, 0, 0, 0, 0, 0, 1, 0, 1)
+
+
+def displacement(d: Deform, x: float, y: float) -> np.ndarray:
+    out = (C.c_double * 6)()
+    lib().twdic_synth_displacement(C.byref(d), x, y, out)
+    return np.array(out[:])
+
+
+# --------------------------------------------------------------------------- device
+class DeviceArray:
+    """A device allocation made through twdic_malloc, freed with the context."""
+
+    def __init__(self, ctx: "Context", shape, dtype):
+        self.ctx = ctx
+        self.shape = tuple(np.atleast_1d(shape).tolist()) if not isinstance(shape, tuple) else shape
+        self.dtype = np.dtype(dtype)
+        self.nbytes = int(np.prod(self.shape)) * self.dtype.itemsize
+        p = C.c_void_p()
+        ctx._check(lib().twdic_malloc(ctx.h, C.byref(p), max(1, self.nbytes)), "twdic_malloc")
+        self.ptr = p.value
+        ctx._allocs.append(self)
+
+    def upload(self, a: np.ndarray) -> "DeviceArray":
+        a = np.ascontiguousarray(a, dtype=self.dtype)
+        assert a.nbytes == self.nbytes, (a.nbytes, self.nbytes)
+        self.ctx._check(lib().twdic_memcpy(self.ctx.h, self.ptr, _ptr(a), self.nbytes, 0), "memcpy h2d")
+        return self
+
+    def download(self) -> np.ndarray:
+        out = np.empty(self.shape, self.dtype)
+        self.ctx._check(lib().twdic_memcpy(self.ctx.h, _ptr(out), self.ptr, self.nbytes, 1), "memcpy d2h")
+        return out
+
+    def fill(self, byte: int) -> "DeviceArray":
+        self.ctx._check(lib().twdic_memset(self.ctx.h, self.ptr, byte, self.nbytes), "memset")
+        return self
+
+
+class Context:
+    """One twdic_ctx (one HIP stream, one device)."""
+
+    def __init__(self, cfg: Config, device: int = 0):
+        self.cfg = cfg
+        h = C.c_void_p()
+        st = lib().twdic_create(C.byref(cfg), device, C.byref(h))
+        if st != OK:
+            raise TwdicError(f"twdic_create failed with status {st}")
+        self.h = h.value
+        self._allocs: list[DeviceArray] = []
+        nx, ny = C.c_int32(), C.c_int32()
+        lib().twdic_num_poi(C.byref(cfg), C.byref(nx), C.byref(ny))
+        self.nx, self.ny = nx.value, ny.value
+
+    def _check(self, st, what=""):
+        if st != OK:
+            msg = lib().twdic_last_error(self.h).decode() if self.h else ""
+            raise TwdicError(f"{what}: status {st}: {msg}")
+
+    def close(self):
+        if self.h:
+            for a in self._allocs:
+                lib().twdic_free(self.h, a.ptr)
+            self._allocs.clear()
+            lib().twdic_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def array(self, shape, dtype) -> DeviceArray:
+        return DeviceArray(self, shape, dtype)
+
+    def sync(self):
+        self._check(lib().twdic_sync(self.h), "sync")
+
+    def set_reference(self, img: np.ndarray):
+        img = np.ascontiguousarray(img, np.uint8)
+        self._check(lib().twdic_set_reference(self.h, _ptr(img), img.shape[1], 0), "set_reference")
+        self.sync()
+
+    def set_target(self, img: np.ndarray):
+        img = np.ascontiguousarray(img, np.uint8)
+        self._check(lib().twdic_set_target(self.h, _ptr(img), img.shape[1], 0), "set_target")
+        self.sync()
+
+    def set_reference_device(self, d: DeviceArray, stride: int):
+        self._check(lib().twdic_set_reference(self.h, d.ptr, stride, 1), "set_reference")
+
+    def set_target_device(self, d: DeviceArray, stride: int):
+        self._check(lib().twdic_set_target(self.h, d.ptr, stride, 1), "set_target")
+
+    def poi_grid(self) -> DeviceArray:
+        d = self.array((self.nx * self.ny, 2), np.int32)
+        self._check(lib().twdic_poi_grid(self.h, d.ptr), "poi_grid")
+        return d
+
+    def fftcc(self, poi: DeviceArray, n: int, u: DeviceArray, v: DeviceArray, zncc: DeviceArray,
+              status: DeviceArray | None = None):
+        self._check(lib().twdic_fftcc(self.h, poi.ptr, n, u.ptr, v.ptr, zncc.ptr,
+                                      status.ptr if status is not None else None), "fftcc")
+
+    def icgn(self, poi: DeviceArray, n: int, u: DeviceArray, v: DeviceArray, p6=None, iters=None,
+             flags=None, dpnorm=None):
+        g = lambda a: a.ptr if a is not None else None  # noqa: E731
+        self._check(lib().twdic_icgn(self.h, poi.ptr, n, u.ptr, v.ptr, g(p6), g(iters), g(flags), g(dpnorm)),
+                    "icgn")
+
+    def frame(self, poi, n, u, v, zncc, p6=None, iters=None, flags=None):
+        g = lambda a: a.ptr if a is not None else None  # noqa: E731
+        self._check(lib().twdic_frame(self.h, poi.ptr, n, u.ptr, v.ptr, zncc.ptr, g(p6), g(iters), g(flags)),
+                    "frame")
+
+    def precompute_target(self):
+        self._check(lib().twdic_precompute_target(self.h), "precompute_target")
+
+    def buffer(self, which: int) -> tuple[int, int]:
+        p, b = C.c_void_p(), C.c_size_t()
+        self._check(lib().twdic_get_buffer(self.h, which, C.byref(p), C.byref(b)), "get_buffer")
+        return p.value, b.value
+
+    def download_buffer(self, which: int, shape, dtype) -> np.ndarray:
+        p, b = self.buffer(which)
+        out = np.empty(shape, dtype)
+        assert out.nbytes <= b
+        self._check(lib().twdic_memcpy(self.h, _ptr(out), p, out.nbytes, 1), "memcpy d2h")
+        return out
+
+    def record(self, slot: int):
+        self._check(lib().twdic_event_record(self.h, slot), "event_record")
+
+    def elapsed_ms(self, a: int, b: int) -> float:
+        ms = C.c_float()
+        self._check(lib().twdic_event_elapsed(self.h, a, b, C.byref(ms)), "event_elapsed")
+        return ms.value
+
+
+def make_config(img_w, img_h, roi_x, roi_y, roi_w, roi_h, sx, sy, gx, gy, mx, my, fft_n, max_iter=20,
+                delta_p=0.001, interp=INTERP_FUSED) -> Config:
+    return Config(img_w, img_h, roi_x, roi_y, roi_w, roi_h, sx, sy, gx, gy, mx, my, fft_n, max_iter,
+                  delta_p, interp)
+
+
+def num_poi(cfg: Config) -> tuple[int, int]:
+    nx, ny = C.c_int32(), C.c_int32()
+    lib().twdic_num_poi(C.byref(cfg), C.byref(nx), C.byref(ny))
+    return nx.value, ny.value

@@ -1,0 +1,273 @@
+// icgn.hip -- first-order IC-GN sub-pixel registration for gfx950, one
+// wavefront per POI, four POIs per 256-thread workgroup.
+//
+// Semantics follow ICGN2D_CPU::ICGN2D_Compute (TW_paDIC_ICGN2D_CPU.cpp:
+// 481-925): first pass at integer target pixels, later passes bicubic; flags
+// DarkSubset / OutofROI / SingularHessian / SingularWarp in the same order;
+// 1-based iteration count; W <- W(p) o W(dp)^-1 by the same closed form
+// (:713-745) in FP32 without contraction; the loop test (:750-751) on the
+// FP32 dp with the norm evaluated in FP64 as the reference's pow() does.
+//
+// Per-iteration work stays in registers:
+//   * each lane walks pixels lane, lane+64, ... of the (2Sy+1)x(2Sx+1) subset;
+//   * one pass forms sum T, sum T^2 and the six sums SD_k T in FP64 (every
+//     FP32 product SD_k T is exact in FP64), so the residual vector is
+//       RHS_k = (R_s/T_s) (sum SD_k T - mean(T) A_k) - B_k
+//     with A_k = sum SD_k and B_k = sum SD_k (R - mean R) precomputed per
+//     reference (k_ref_state).  This equals the reference's three passes
+//     (:793-826) algebraically and is more accurate;
+//   * wavefront butterflies reduce the 8 sums (all lanes end bit-identical);
+//   * the 6x6 solve uses the per-POI FP64 Cholesky factor.
+#include "twdic_internal.h"
+
+#include <cfloat>
+
+namespace twdic {
+
+namespace {
+
+__device__ __forceinline__ double wave_sum(double v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    return v;
+}
+
+// Warp coordinate exactly as the CPU path evaluates it (:618-619, :761-762):
+// ((POI + W0*dx) + W1*dy) + W2, every product and sum rounded to FP32.
+__device__ __forceinline__ float warp_coord(int poi, float w0, float w1, float w2, float dx, float dy)
+{
+    return __fadd_rn(__fadd_rn(__fadd_rn((float)poi, __fmul_rn(w0, dx)), __fmul_rn(w1, dy)), w2);
+}
+
+// Bicubic value from the materialised LUT cell (16 floats, 4k+l ~ y^k x^l).
+__device__ __forceinline__ float interp_lut(const float* __restrict__ lut, size_t cell, float tx, float ty)
+{
+    const float4* c = reinterpret_cast<const float4*>(lut + cell * 16);
+    float4 r0 = c[0], r1 = c[1], r2 = c[2], r3 = c[3];
+    float a0 = fmaf(fmaf(fmaf(r0.w, tx, r0.z), tx, r0.y), tx, r0.x);
+    float a1 = fmaf(fmaf(fmaf(r1.w, tx, r1.z), tx, r1.y), tx, r1.x);
+    float a2 = fmaf(fmaf(fmaf(r2.w, tx, r2.z), tx, r2.y), tx, r2.x);
+    float a3 = fmaf(fmaf(fmaf(r3.w, tx, r3.z), tx, r3.y), tx, r3.x);
+    return fmaf(fmaf(fmaf(a3, ty, a2), ty, a1), ty, a0);
+}
+
+// The same bicubic interpolant evaluated from the 4x4 pixel neighbourhood:
+// the LUT polynomial with central-difference corner derivatives is the
+// tensor-product Catmull-Rom spline.  Differences to the cell's top-left
+// pixel keep constant regions exact.
+__device__ __forceinline__ void catmull_rom_w(float t, float w[4])
+{
+    float t2 = t * t;
+    w[0] = 0.5f * (t * (2.f * t - t2 - 1.f));
+    w[1] = 0.5f * (t2 * (3.f * t - 5.f) + 2.f);
+    w[2] = 0.5f * (t * ((4.f - 3.f * t) * t + 1.f));
+    w[3] = 0.5f * (t2 * (t - 1.f));
+}
+
+__device__ __forceinline__ float interp_fused(const uint8_t* __restrict__ tar, int stride, int ix, int iy,
+                                              float tx, float ty)
+{
+    const uint8_t* b = tar + (size_t)(iy - 1) * stride + (ix - 1);
+    float wx[4], wy[4];
+    catmull_rom_w(tx, wx);
+    catmull_rom_w(ty, wy);
+    const int c = b[stride + 1];
+    float acc = 0.f;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        const uint8_t* row = b + r * stride;
+        float s = wx[0] * (float)(row[0] - c);
+        s = fmaf(wx[1], (float)(row[1] - c), s);
+        s = fmaf(wx[2], (float)(row[2] - c), s);
+        s = fmaf(wx[3], (float)(row[3] - c), s);
+        acc = fmaf(wy[r], s, acc);
+    }
+    return (float)c + acc;
+}
+
+// W(p) o W(dp)^-1 and p from W: TW_paDIC_ICGN2D_CPU.cpp:713-745 / :875-907,
+// in FP32 with every operation rounded separately (no contraction).
+__device__ __forceinline__ void warp_update(float P[6], const float d[6], float den, float W[6])
+{
+#pragma clang fp contract(off)
+    float w00 = ((1 + P[1]) * (1 + d[5]) - P[2] * d[4]) / den;
+    float w01 = (P[2] * (1 + d[1]) - (1 + P[1]) * d[2]) / den;
+    float w02 = P[0] + (P[2] * (d[0] * d[4] - d[3] - d[3] * d[1]) - (1 + P[1]) * (d[0] * d[5] + d[0] - d[2] * d[3])) / den;
+    float w10 = (P[4] * (1 + d[5]) - (1 + P[5]) * d[4]) / den;
+    float w11 = ((1 + P[5]) * (1 + d[1]) - P[4] * d[2]) / den;
+    float w12 = P[3] + ((1 + P[5]) * (d[0] * d[4] - d[3] - d[3] * d[1]) - P[4] * (d[0] * d[5] + d[0] - d[2] * d[3])) / den;
+    W[0] = w00; W[1] = w01; W[2] = w02; W[3] = w10; W[4] = w11; W[5] = w12;
+    P[0] = w02;
+    P[1] = w00 - 1;
+    P[2] = w01;
+    P[3] = w12;
+    P[4] = w10;
+    P[5] = w11 - 1;
+}
+
+__device__ __forceinline__ float warp_den(const float d[6])
+{
+#pragma clang fp contract(off)
+    return (1 + d[1]) * (1 + d[5]) - d[2] * d[4];
+}
+
+// Loop test quantity (:751): products in FP32, squares and sum in FP64.
+__device__ __forceinline__ double conv_norm(const float d[6], int Sx, int Sy)
+{
+    double t0 = d[0];
+    double t1 = __fmul_rn(d[1], (float)Sx);
+    double t2 = __fmul_rn(d[2], (float)Sy);
+    double t3 = d[3];
+    double t4 = __fmul_rn(d[4], (float)Sx);
+    double t5 = __fmul_rn(d[5], (float)Sy);
+    return sqrt(t0 * t0 + t1 * t1 + t2 * t2 + t3 * t3 + t4 * t4 + t5 * t5);
+}
+
+}  // namespace
+
+template <int INTERP>
+__global__ __launch_bounds__(256) void k_icgn(IcgnParams P)
+{
+    const int lane = threadIdx.x & 63;
+    const int poi = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (poi >= P.n) return;
+    const int py = P.poi[2 * poi], px = P.poi[2 * poi + 1];
+    const int Sx = P.Sx, Sy = P.Sy, Wsub = P.Wsub, s = P.s;
+    const int sx0 = P.sx0, sy0 = P.sy0, rw = P.rw, rh = P.rh;
+
+    float Pp[6] = {P.u[poi], 0.f, 0.f, P.v[poi], 0.f, 0.f};
+    float Wm[6] = {1.f + Pp[1], Pp[2], Pp[0], Pp[4], 1.f + Pp[5], Pp[3]};  // :605-607
+    int iters = 0;                 // 0: leave the caller's value untouched
+    int flag = TWDIC_ICGN_SUCCESS;
+    float dpn = __builtin_nanf("");
+
+    const double* rs = P.refstate + (size_t)poi * kRefStateStride;
+    const int refflag = (int)rs[RS_FLAG];
+    if (refflag == TWDIC_ICGN_DARK_SUBSET || refflag == TWDIC_ICGN_OUT_OF_ROI) {
+        flag = refflag;
+    } else {
+        double L[21], A[6], B[6];
+        const double normR = rs[RS_NORM], meanR = rs[RS_MEAN];
+#pragma unroll
+        for (int i = 0; i < 21; ++i) L[i] = rs[RS_L + i];
+#pragma unroll
+        for (int i = 0; i < 6; ++i) {
+            A[i] = rs[RS_A + i];
+            B[i] = centred(rs[RS_B + i], meanR, A[i]);   // sum SD_k (R - mean R)
+        }
+        const int q64 = 64 / Wsub, r64 = 64 - q64 * Wsub;
+        const int l0 = lane / Wsub, m0 = lane - l0 * Wsub;
+
+        for (int it = 0;; ++it) {
+            if (it >= 1) iters = it + 1;                       // ++iNumIterations (:753)
+            double st = 0.0, stt = 0.0, sd0 = 0.0, sd1 = 0.0, sd2 = 0.0, sd3 = 0.0, sd4 = 0.0, sd5 = 0.0;
+            bool out = false;
+            int l = l0, m = m0;
+            for (int p = lane; p < s; p += 64) {
+                const float dx = (float)(m - Sx), dy = (float)(l - Sy);
+                const float wx = warp_coord(px, Wm[0], Wm[1], Wm[2], dx, dy);
+                const float wy = warp_coord(py, Wm[3], Wm[4], Wm[5], dx, dy);
+                const int ix = (int)wx, iy = (int)wy;          // truncation, as int()
+                const bool in = ix >= sx0 && iy >= sy0 && ix < sx0 + rw && iy < sy0 + rh;
+                out |= !in;
+                float T = 0.f;
+                if (in) {
+                    if (it == 0) {
+                        T = (float)P.tar[(size_t)iy * P.stride + ix];          // :629
+                    } else if (ix == sx0 + rw - 1 || iy == sy0 + rh - 1) {
+                        T = 0.f;   // the CPU LUT's last row/column is zero (:643-699)
+                    } else {
+                        const float tx = wx - (float)ix, ty = wy - (float)iy;   // :771-772
+                        if (INTERP == TWDIC_INTERP_LUT)
+                            T = interp_lut(P.lut, (size_t)(iy - sy0) * rw + (ix - sx0), tx, ty);
+                        else
+                            T = interp_fused(P.tar, P.stride, ix, iy, tx, ty);
+                    }
+                }
+                const size_t gi = (size_t)(py - Sy + l - sy0) * rw + (px - Sx + m - sx0);
+                const double gx = P.rx[gi], gy = P.ry[gi];
+                const double Td = T;
+                st += Td;
+                stt = fma(Td, Td, stt);
+                const double a = gx * Td, b = gy * Td;
+                sd0 += a;
+                sd1 = fma(a, (double)dx, sd1);
+                sd2 = fma(a, (double)dy, sd2);
+                sd3 += b;
+                sd4 = fma(b, (double)dx, sd4);
+                sd5 = fma(b, (double)dy, sd5);
+                m += r64; l += q64;
+                if (m >= Wsub) { m -= Wsub; l += 1; }
+            }
+            if (__any(out)) { flag = TWDIC_ICGN_OUT_OF_ROI; break; }   // :632-635, :787-790
+            st = wave_sum(st); stt = wave_sum(stt);
+            sd0 = wave_sum(sd0); sd1 = wave_sum(sd1); sd2 = wave_sum(sd2);
+            sd3 = wave_sum(sd3); sd4 = wave_sum(sd4); sd5 = wave_sum(sd5);
+            double meanT, normT;
+            mean_spread(st, stt, (double)s, meanT, normT);
+            if (fabsf((float)meanT) <= FLT_EPSILON) { flag = TWDIC_ICGN_DARK_SUBSET; break; }   // :638-640
+            if ((float)normT <= FLT_EPSILON) { flag = TWDIC_ICGN_DARK_SUBSET; break; }          // :651-653
+            if (it == 0 && refflag == TWDIC_ICGN_SINGULAR_HESSIAN) {   // :688-693
+                flag = TWDIC_ICGN_SINGULAR_HESSIAN;
+                break;
+            }
+            const double nor = normR / normT;
+            // RHS_k = sum SD_k ((R_s/T_s) (T - mean T) - (R - mean R))   (:657-667)
+            double r[6] = {__fma_rn(nor, centred(sd0, meanT, A[0]), -B[0]),
+                           __fma_rn(nor, centred(sd1, meanT, A[1]), -B[1]),
+                           __fma_rn(nor, centred(sd2, meanT, A[2]), -B[2]),
+                           __fma_rn(nor, centred(sd3, meanT, A[3]), -B[3]),
+                           __fma_rn(nor, centred(sd4, meanT, A[4]), -B[4]),
+                           __fma_rn(nor, centred(sd5, meanT, A[5]), -B[5])};
+            // L y = r, L^T x = y (spotrs 'L', :683, :853) in FP64
+#pragma unroll
+            for (int i = 0; i < 6; ++i) {
+                double v = r[i];
+#pragma unroll
+                for (int k = 0; k < i; ++k) v -= L[i * (i + 1) / 2 + k] * r[k];
+                r[i] = v / L[i * (i + 1) / 2 + i];
+            }
+#pragma unroll
+            for (int i = 5; i >= 0; --i) {
+                double v = r[i];
+#pragma unroll
+                for (int k = i + 1; k < 6; ++k) v -= L[k * (k + 1) / 2 + i] * r[k];
+                r[i] = v / L[i * (i + 1) / 2 + i];
+            }
+            float d[6];
+#pragma unroll
+            for (int i = 0; i < 6; ++i) d[i] = (float)r[i];
+            const float den = warp_den(d);
+            if (fabsf(den) <= FLT_EPSILON) { flag = TWDIC_ICGN_SINGULAR_WARP; break; }
+            warp_update(Pp, d, den, Wm);
+            dpn = (float)conv_norm(d, Sx, Sy);
+            if (it == 0) iters = 1;                                    // :749
+            if (!(iters < P.max_iter && conv_norm(d, Sx, Sy) >= (double)P.delta_p)) break;
+        }
+    }
+    if (lane == 0) {
+        P.u[poi] = Pp[0];
+        P.v[poi] = Pp[3];
+        if (P.p6) {
+#pragma unroll
+            for (int i = 0; i < 6; ++i) P.p6[(size_t)poi * 6 + i] = Pp[i];
+        }
+        if (P.iters && iters > 0) P.iters[poi] = iters;
+        if (P.flags) P.flags[poi] = (uint8_t)flag;
+        if (P.dpnorm) P.dpnorm[poi] = dpn;
+    }
+}
+
+hipError_t launch_icgn(const IcgnParams& p, int interp, hipStream_t st)
+{
+    if (p.n <= 0) return hipSuccess;
+    dim3 grid((p.n + 3) / 4);
+    if (interp == TWDIC_INTERP_FUSED)
+        hipLaunchKernelGGL(k_icgn<TWDIC_INTERP_FUSED>, grid, dim3(256), 0, st, p);
+    else
+        hipLaunchKernelGGL(k_icgn<TWDIC_INTERP_LUT>, grid, dim3(256), 0, st, p);
+    return hipGetLastError();
+}
+
+}  // namespace twdic

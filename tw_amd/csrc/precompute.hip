@@ -1,0 +1,250 @@
+// precompute.hip -- POI grid, central-difference gradients, bicubic LUT and
+// the per-reference IC-GN state (FP64 Hessian + Cholesky), for gfx950.
+//
+// All gradient and LUT values are exact in FP32 (half/quarter integers and
+// sums of at most 16 integer multiples of them, below 2^14), so these kernels
+// are bit-identical to the CPU path in any summation order.
+#include "twdic_internal.h"
+
+#include <cfloat>
+
+namespace twdic {
+
+// ---------------------------------------------------------------------------
+// POI grid: TW_utils.cpp:29-49 / TW_cuUtils.cu:116-134 (y first).
+// ---------------------------------------------------------------------------
+__global__ void k_poi_grid(int32_t* poi, int start_x, int start_y, int nx, int ny, int mx, int my,
+                           int sx, int sy, int gx, int gy)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nx * ny) return;
+    int iy = i / nx, ix = i - iy * nx;
+    int2 yx = make_int2(start_y + my + sy + iy * gy, start_x + mx + sx + ix * gx);
+    reinterpret_cast<int2*>(poi)[i] = yx;
+}
+
+hipError_t launch_poi_grid(int32_t* poi, int start_x, int start_y, int nx, int ny, int mx, int my,
+                           int sx, int sy, int gx, int gy, hipStream_t st)
+{
+    int n = nx * ny;
+    hipLaunchKernelGGL(k_poi_grid, dim3((n + 255) / 256), dim3(256), 0, st, poi, start_x, start_y,
+                       nx, ny, mx, my, sx, sy, gx, gy);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Gradients: GradientXY_2Images_s (TW_utils.cpp:260-273), one image.
+// Thread per pixel, x fastest; stores are fully coalesced.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_gradients(const uint8_t* __restrict__ img, int stride,
+                                                   int sx0, int sy0, int rw, int rh,
+                                                   float* __restrict__ gx, float* __restrict__ gy,
+                                                   float* __restrict__ gxy)
+{
+    int x = blockIdx.x * 64 + (threadIdx.x & 63);
+    int y = blockIdx.y * 4 + (threadIdx.x >> 6);
+    if (x >= rw || y >= rh) return;
+    const uint8_t* c = img + (size_t)(y + sy0) * stride + (x + sx0);
+    int l = c[-1], r = c[1], u = c[-stride], d = c[stride];
+    size_t o = (size_t)y * rw + x;
+    gx[o] = 0.5f * (float)(r - l);
+    gy[o] = 0.5f * (float)(d - u);
+    if (gxy) {
+        int ur = c[-stride + 1], ul = c[-stride - 1], dr = c[stride + 1], dl = c[stride - 1];
+        gxy[o] = 0.25f * (float)(dr - ur - dl + ul);
+    }
+}
+
+hipError_t launch_gradients(const uint8_t* img, int stride, int sx0, int sy0, int rw, int rh,
+                            float* gx, float* gy, float* gxy, hipStream_t st)
+{
+    dim3 grid((rw + 63) / 64, (rh + 3) / 4);
+    hipLaunchKernelGGL(k_gradients, grid, dim3(256), 0, st, img, stride, sx0, sy0, rw, rh, gx, gy, gxy);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Bicubic LUT: BicubicCoefficients_s (TW_utils.cpp:604-700), fused with the
+// target gradients: tau (corner values, x/y/xy central differences) is built
+// straight from the 4x4 pixel neighbourhood, alpha = M tau with the standard
+// bicubic matrix (TW_utils.cpp:622-638) expanded by hand.  Layout per pixel:
+// 16 contiguous floats, element 4k+l multiplies y^k x^l.  The last ROI row and
+// column are written as zeros (the CPU buffer is calloc-zeroed there).
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_bicubic_lut(const uint8_t* __restrict__ tar, int stride,
+                                                     int sx0, int sy0, int rw, int rh,
+                                                     float* __restrict__ lut)
+{
+    int x = blockIdx.x * 64 + (threadIdx.x & 63);
+    int y = blockIdx.y * 4 + (threadIdx.x >> 6);
+    if (x >= rw || y >= rh) return;
+    float4* out = reinterpret_cast<float4*>(lut + ((size_t)y * rw + x) * 16);
+    if (x == rw - 1 || y == rh - 1) {
+        float4 z = make_float4(0.f, 0.f, 0.f, 0.f);
+        out[0] = z; out[1] = z; out[2] = z; out[3] = z;
+        return;
+    }
+    // p[r][c] = T(y0 - 1 + r, x0 - 1 + c), r,c in 0..3
+    const uint8_t* b = tar + (size_t)(y + sy0 - 1) * stride + (x + sx0 - 1);
+    float p[4][4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r)
+#pragma unroll
+        for (int c = 0; c < 4; ++c) p[r][c] = (float)b[r * stride + c];
+    // corner values f, x/y derivatives, cross derivatives (TW_utils.cpp:265-271)
+    float f00 = p[1][1], f10 = p[1][2], f01 = p[2][1], f11 = p[2][2];
+    float x00 = 0.5f * (p[1][2] - p[1][0]), x10 = 0.5f * (p[1][3] - p[1][1]);
+    float x01 = 0.5f * (p[2][2] - p[2][0]), x11 = 0.5f * (p[2][3] - p[2][1]);
+    float y00 = 0.5f * (p[2][1] - p[0][1]), y10 = 0.5f * (p[2][2] - p[0][2]);
+    float y01 = 0.5f * (p[3][1] - p[1][1]), y11 = 0.5f * (p[3][2] - p[1][2]);
+    float c00 = 0.25f * (p[2][2] - p[0][2] - p[2][0] + p[0][0]);
+    float c10 = 0.25f * (p[2][3] - p[0][3] - p[2][1] + p[0][1]);
+    float c01 = 0.25f * (p[3][2] - p[1][2] - p[3][0] + p[1][0]);
+    float c11 = 0.25f * (p[3][3] - p[1][3] - p[3][1] + p[1][1]);
+    // tau = [f00 f10 f01 f11 | x00 x10 x01 x11 | y00 y10 y01 y11 | c00 c10 c01 c11]
+    float a[16];
+    a[0] = f00;
+    a[1] = x00;
+    a[2] = -3.f * f00 + 3.f * f10 - 2.f * x00 - x10;
+    a[3] = 2.f * f00 - 2.f * f10 + x00 + x10;
+    a[4] = y00;
+    a[5] = c00;
+    a[6] = -3.f * y00 + 3.f * y10 - 2.f * c00 - c10;
+    a[7] = 2.f * y00 - 2.f * y10 + c00 + c10;
+    a[8] = -3.f * f00 + 3.f * f01 - 2.f * y00 - y01;
+    a[9] = -3.f * x00 + 3.f * x01 - 2.f * c00 - c01;
+    a[10] = 9.f * f00 - 9.f * f10 - 9.f * f01 + 9.f * f11 + 6.f * x00 + 3.f * x10 - 6.f * x01 - 3.f * x11
+          + 6.f * y00 - 6.f * y10 + 3.f * y01 - 3.f * y11 + 4.f * c00 + 2.f * c10 + 2.f * c01 + c11;
+    a[11] = -6.f * f00 + 6.f * f10 + 6.f * f01 - 6.f * f11 - 3.f * x00 - 3.f * x10 + 3.f * x01 + 3.f * x11
+          - 4.f * y00 + 4.f * y10 - 2.f * y01 + 2.f * y11 - 2.f * c00 - 2.f * c10 - c01 - c11;
+    a[12] = 2.f * f00 - 2.f * f01 + y00 + y01;
+    a[13] = 2.f * x00 - 2.f * x01 + c00 + c01;
+    a[14] = -6.f * f00 + 6.f * f10 + 6.f * f01 - 6.f * f11 - 4.f * x00 - 2.f * x10 + 4.f * x01 + 2.f * x11
+          - 3.f * y00 + 3.f * y10 - 3.f * y01 + 3.f * y11 - 2.f * c00 - c10 - 2.f * c01 - c11;
+    a[15] = 4.f * f00 - 4.f * f10 - 4.f * f01 + 4.f * f11 + 2.f * x00 + 2.f * x10 - 2.f * x01 - 2.f * x11
+          + 2.f * y00 - 2.f * y10 + 2.f * y01 - 2.f * y11 + c00 + c10 + c01 + c11;
+    out[0] = make_float4(a[0], a[1], a[2], a[3]);
+    out[1] = make_float4(a[4], a[5], a[6], a[7]);
+    out[2] = make_float4(a[8], a[9], a[10], a[11]);
+    out[3] = make_float4(a[12], a[13], a[14], a[15]);
+}
+
+hipError_t launch_bicubic_lut(const uint8_t* tar, int stride, int sx0, int sy0, int rw, int rh,
+                              float* lut, hipStream_t st)
+{
+    dim3 grid((rw + 63) / 64, (rh + 3) / 4);
+    hipLaunchKernelGGL(k_bicubic_lut, grid, dim3(256), 0, st, tar, stride, sx0, sy0, rw, rh, lut);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Per-reference state, one wavefront per POI (built once per reference and
+// POI set; the reference rebuilds it on m_isRefImgUpdated,
+// TW_paDIC_cuICGN2D.cu:883-903).  Sums follow ICGN2D_Compute's precompute
+// (TW_paDIC_ICGN2D_CPU.cpp:516-595): SD = [Rx, Rx dx, Rx dy, Ry, Ry dx, Ry dy],
+// H = sum SD^T SD.  Every SD is a half-integer multiple and every product is
+// a multiple of 1/4 below 2^24, so the FP64 sums here are exact and
+// independent of order.  The Cholesky factorisation is FP64.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double wave_sum_d(double v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    return v;
+}
+
+__global__ __launch_bounds__(256) void k_ref_state(IcgnParams P)
+{
+    const int lane = threadIdx.x & 63;
+    const int poi = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (poi >= P.n) return;
+    const int py = P.poi[2 * poi], px = P.poi[2 * poi + 1];
+    double* rs = P.refstate + (size_t)poi * kRefStateStride;
+    // subsets that leave the ROI are flagged by the iteration kernel; keep the
+    // state well defined and touch no memory outside the ROI
+    if (px - P.Sx < P.sx0 || py - P.Sy < P.sy0 || px + P.Sx >= P.sx0 + P.rw ||
+        py + P.Sy >= P.sy0 + P.rh) {
+        if (lane < kRefStateStride) rs[lane] = (lane == RS_FLAG) ? (double)TWDIC_ICGN_OUT_OF_ROI : 0.0;
+        return;
+    }
+    double h[21], a[6], br[6];
+#pragma unroll
+    for (int i = 0; i < 21; ++i) h[i] = 0.0;
+#pragma unroll
+    for (int i = 0; i < 6; ++i) { a[i] = 0.0; br[i] = 0.0; }
+    long long sr = 0, srr = 0;
+    const int W = P.Wsub, s = P.s;
+    const int q64 = kWave / W, r64 = kWave - q64 * W;
+    int l = lane / W, m = lane - l * W;
+    for (int p = lane; p < s; p += kWave) {
+        int gy = py - P.Sy + l, gxp = px - P.Sx + m;
+        int R = P.ref[(size_t)gy * P.stride + gxp];
+        size_t gi = (size_t)(gy - P.sy0) * P.rw + (gxp - P.sx0);
+        double rx = P.rx[gi], ry = P.ry[gi];
+        double dx = (double)(m - P.Sx), dy = (double)(l - P.Sy);
+        double sd[6] = {rx, rx * dx, rx * dy, ry, ry * dx, ry * dy};
+        int t = 0;
+#pragma unroll
+        for (int k = 0; k < 6; ++k)
+#pragma unroll
+            for (int q = 0; q <= k; ++q) h[t++] += sd[k] * sd[q];
+#pragma unroll
+        for (int k = 0; k < 6; ++k) { a[k] += sd[k]; br[k] += sd[k] * (double)R; }
+        sr += R;
+        srr += (long long)R * R;
+        m += r64; l += q64;
+        if (m >= W) { m -= W; l += 1; }
+    }
+#pragma unroll
+    for (int i = 0; i < 21; ++i) h[i] = wave_sum_d(h[i]);
+#pragma unroll
+    for (int i = 0; i < 6; ++i) { a[i] = wave_sum_d(a[i]); br[i] = wave_sum_d(br[i]); }
+    double dsr = wave_sum_d((double)sr), dsrr = wave_sum_d((double)srr);  // exact (< 2^53)
+
+    double mean, spread;
+    mean_spread(dsr, dsrr, (double)s, mean, spread);
+    double flag = 0.0;
+    // TW_paDIC_ICGN2D_CPU.cpp:580-595: zero mean or zero spread -> DarkSubset.
+    // Both sums are exact integers, so the test is exact: all-zero or uniform.
+    if (dsr == 0.0 || dsrr * (double)s == dsr * dsr) flag = (double)TWDIC_ICGN_DARK_SUBSET;
+    double L[21];
+#pragma unroll
+    for (int i = 0; i < 21; ++i) L[i] = h[i];
+    if (flag == 0.0) {
+        // Cholesky (spotrf 'L', TW_paDIC_ICGN2D_CPU.cpp:682,688-693) in FP64
+        for (int j = 0; j < 6; ++j) {
+            int jj = j * (j + 1) / 2;
+            double d = L[jj + j];
+            for (int k = 0; k < j; ++k) d -= L[jj + k] * L[jj + k];
+            if (!(d > 0.0)) { flag = (double)TWDIC_ICGN_SINGULAR_HESSIAN; break; }
+            d = sqrt(d);
+            L[jj + j] = d;
+            for (int i = j + 1; i < 6; ++i) {
+                int ii = i * (i + 1) / 2;
+                double v = L[ii + j];
+                for (int k = 0; k < j; ++k) v -= L[ii + k] * L[jj + k];
+                L[ii + j] = v / d;
+            }
+        }
+    }
+    if (lane == 0) {
+#pragma unroll
+        for (int i = 0; i < 21; ++i) rs[RS_L + i] = L[i];
+#pragma unroll
+        for (int i = 0; i < 6; ++i) {
+            rs[RS_A + i] = a[i];
+            rs[RS_B + i] = br[i];
+        }
+        rs[RS_NORM] = spread;
+        rs[RS_FLAG] = flag;
+        rs[RS_MEAN] = mean;
+    }
+}
+
+hipError_t launch_ref_state(const IcgnParams& p, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_ref_state, dim3((p.n + 3) / 4), dim3(256), 0, st, p);
+    return hipGetLastError();
+}
+
+}  // namespace twdic

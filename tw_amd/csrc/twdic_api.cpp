@@ -1,0 +1,406 @@
+// twdic_api.cpp -- the extern "C" boundary (include/twdic.h): context
+// lifetime, image upload, precompute orchestration and launches.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <new>
+
+#include "twdic_internal.h"
+
+using namespace twdic;
+
+namespace {
+
+twdic_status fail(twdic_ctx* c, twdic_status st, const char* what)
+{
+    if (c) c->err = what;
+    return st;
+}
+
+twdic_status hip_fail(twdic_ctx* c, hipError_t e, const char* where)
+{
+    if (c) {
+        c->err = where;
+        c->err += ": ";
+        c->err += hipGetErrorString(e);
+    }
+    return TWDIC_EHIP;
+}
+
+#define TW_HIP(ctx, expr)                                         \
+    do {                                                          \
+        hipError_t e_ = (expr);                                   \
+        if (e_ != hipSuccess) return hip_fail((ctx), e_, #expr);  \
+    } while (0)
+
+// Fftcc2D::recomputeNumPOI (TW_paDIC_FFTCC2D.cpp:88-94): float division,
+// floor, truncation.
+void num_poi(const twdic_config& c, int* nx, int* ny)
+{
+    *nx = (int)std::floor((float)(c.roi_w - c.sx * 2 - c.mx * 2) / (float)c.gx) + 1;
+    *ny = (int)std::floor((float)(c.roi_h - c.sy * 2 - c.my * 2) / (float)c.gy) + 1;
+}
+
+const char* validate(const twdic_config& c)
+{
+    if (c.img_w <= 0 || c.img_h <= 0) return "image size must be positive";
+    if (c.roi_w < 2 || c.roi_h < 2) return "ROI must be at least 2x2";
+    // the gradients need a one-pixel frame around the ROI (TW_utils.cpp:246-257,
+    // with the reference's off-by-one corrected)
+    if (c.roi_x < 1 || c.roi_y < 1 || c.roi_x + c.roi_w + 1 > c.img_w || c.roi_y + c.roi_h + 1 > c.img_h)
+        return "Error: Not enough boundary pixels for gradients!";
+    if (c.sx < 1 || c.sy < 1 || c.sx > 60 || c.sy > 60) return "subset half size must be in [1, 60]";
+    if (c.gx < 1 || c.gy < 1) return "grid spacing must be >= 1";
+    if (c.mx < 0 || c.my < 0) return "margins must be >= 0";
+    if (c.fft_n < 2 || c.fft_n > kMaxFftN || (c.fft_n & 1)) return "fft_n must be even and in [2, 64]";
+    if (c.max_iter < 1) return "max_iter must be >= 1";
+    if (!(c.delta_p >= 0.f)) return "delta_p must be >= 0";
+    if (c.interp != TWDIC_INTERP_LUT && c.interp != TWDIC_INTERP_FUSED) return "unknown interp";
+    int nx, ny;
+    num_poi(c, &nx, &ny);
+    if (nx <= 0 || ny <= 0) return "Number of POIs is below 0!";
+    return nullptr;
+}
+
+twdic_status upload(twdic_ctx* c, uint8_t* dst, const uint8_t* img, int stride, int on_device)
+{
+    const twdic_config& g = c->cfg;
+    if (!img || stride < g.img_w) return fail(c, TWDIC_EINVAL, "image pointer/stride invalid");
+    if (on_device) {
+        TW_HIP(c, hipMemcpy2DAsync(dst, c->dstride, img, stride, g.img_w, g.img_h, hipMemcpyDeviceToDevice, c->stream));
+        return TWDIC_OK;
+    }
+    // Host image: copy into pinned staging, then an async DMA.  The staging
+    // buffer is reused only after the previous DMA out of it has finished.
+    size_t bytes = (size_t)g.img_w * g.img_h;
+    if (c->h_stage_bytes < bytes) {
+        if (c->h_stage) {
+            TW_HIP(c, hipStreamSynchronize(c->stream));
+            (void)hipHostFree(c->h_stage);
+            c->h_stage = nullptr;
+        }
+        TW_HIP(c, hipHostMalloc((void**)&c->h_stage, bytes, hipHostMallocDefault));
+        c->h_stage_bytes = bytes;
+    } else {
+        TW_HIP(c, hipEventSynchronize(c->ev[15]));
+    }
+    for (int y = 0; y < g.img_h; ++y) std::memcpy(c->h_stage + (size_t)y * g.img_w, img + (size_t)y * stride, g.img_w);
+    TW_HIP(c, hipMemcpy2DAsync(dst, c->dstride, c->h_stage, g.img_w, g.img_w, g.img_h, hipMemcpyHostToDevice, c->stream));
+    TW_HIP(c, hipEventRecord(c->ev[15], c->stream));
+    return TWDIC_OK;
+}
+
+twdic_status ensure_ref_grads(twdic_ctx* c)
+{
+    if (!c->ref_loaded) return fail(c, TWDIC_EINVAL, "reference image not set");
+    if (c->ref_grads_valid) return TWDIC_OK;
+    const twdic_config& g = c->cfg;
+    TW_HIP(c, launch_gradients(c->d_ref, c->dstride, g.roi_x, g.roi_y, g.roi_w, g.roi_h, c->d_rx, c->d_ry,
+                               nullptr, c->stream));
+    c->ref_grads_valid = true;
+    return TWDIC_OK;
+}
+
+IcgnParams icgn_params(twdic_ctx* c, const int32_t* poi, int n, float* u, float* v, float* p6,
+                       int32_t* iters, uint8_t* flags, float* dpnorm)
+{
+    const twdic_config& g = c->cfg;
+    IcgnParams p{};
+    p.ref = c->d_ref; p.tar = c->d_tar; p.stride = c->dstride;
+    p.sx0 = g.roi_x; p.sy0 = g.roi_y; p.rw = g.roi_w; p.rh = g.roi_h;
+    p.Sx = g.sx; p.Sy = g.sy; p.Wsub = 2 * g.sx + 1; p.Hsub = 2 * g.sy + 1; p.s = p.Wsub * p.Hsub;
+    p.max_iter = g.max_iter; p.delta_p = g.delta_p;
+    p.rx = c->d_rx; p.ry = c->d_ry; p.lut = c->d_lut;
+    p.refstate = c->d_refstate;
+    p.poi = poi; p.n = n;
+    p.u = u; p.v = v; p.p6 = p6; p.iters = iters; p.flags = flags; p.dpnorm = dpnorm;
+    return p;
+}
+
+}  // namespace
+
+extern "C" {
+
+twdic_status twdic_num_poi(const twdic_config* cfg, int32_t* nx, int32_t* ny)
+{
+    if (!cfg || !nx || !ny) return TWDIC_EINVAL;
+    int a, b;
+    num_poi(*cfg, &a, &b);
+    *nx = a; *ny = b;
+    return (a > 0 && b > 0) ? TWDIC_OK : TWDIC_EINVAL;
+}
+
+twdic_status twdic_create(const twdic_config* cfg, int device, twdic_ctx** out)
+{
+    if (!cfg || !out) return TWDIC_EINVAL;
+    *out = nullptr;
+    if (validate(*cfg)) return TWDIC_EINVAL;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0 || device < 0 || device >= ndev) return TWDIC_ENODEV;
+    twdic_ctx* c = new (std::nothrow) twdic_ctx();
+    if (!c) return TWDIC_ENOMEM;
+    c->cfg = *cfg;
+    c->device = device;
+    num_poi(*cfg, &c->nx, &c->ny);
+    c->dstride = (cfg->img_w + 63) & ~63;
+    twdic_status st = TWDIC_OK;
+    auto bail = [&](hipError_t e, const char* w) {
+        hip_fail(c, e, w);
+        st = (e == hipErrorOutOfMemory) ? TWDIC_ENOMEM : TWDIC_EHIP;
+    };
+    hipError_t e;
+    if ((e = hipSetDevice(device)) != hipSuccess) bail(e, "hipSetDevice");
+    if (st == TWDIC_OK && (e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking)) != hipSuccess) bail(e, "hipStreamCreate");
+    const size_t img = (size_t)c->dstride * cfg->img_h;
+    const size_t roi = (size_t)cfg->roi_w * cfg->roi_h;
+    struct { void** p; size_t b; } allocs[] = {
+        {(void**)&c->d_ref, img}, {(void**)&c->d_tar, img},
+        {(void**)&c->d_rx, roi * 4}, {(void**)&c->d_ry, roi * 4},
+        {(void**)&c->d_tx, roi * 4}, {(void**)&c->d_ty, roi * 4}, {(void**)&c->d_txy, roi * 4},
+        {(void**)&c->d_lut, roi * 64}, {(void**)&c->d_fft_tw, sizeof(float2) * kMaxFftN},
+    };
+    for (auto& a : allocs) {
+        if (st != TWDIC_OK) break;
+        if ((e = hipMalloc(a.p, a.b)) != hipSuccess) bail(e, "hipMalloc");
+        else if ((e = hipMemset(*a.p, 0, a.b)) != hipSuccess) bail(e, "hipMemset");
+    }
+    for (int i = 0; i < 16 && st == TWDIC_OK; ++i)
+        if ((e = hipEventCreate(&c->ev[i])) != hipSuccess) bail(e, "hipEventCreate");
+    if (st == TWDIC_OK) {
+        // twiddles exp(-2 pi i e / N), rounded from double
+        float2 tw[kMaxFftN];
+        const int N = cfg->fft_n;
+        for (int k = 0; k < N; ++k) {
+            double a = -2.0 * M_PI * (double)k / (double)N;
+            tw[k] = make_float2((float)std::cos(a), (float)std::sin(a));
+        }
+        if ((e = hipMemcpy(c->d_fft_tw, tw, sizeof(float2) * N, hipMemcpyHostToDevice)) != hipSuccess)
+            bail(e, "hipMemcpy twiddles");
+    }
+    if (st != TWDIC_OK) {
+        twdic_destroy(c);
+        return st;
+    }
+    *out = c;
+    return TWDIC_OK;
+}
+
+twdic_status twdic_destroy(twdic_ctx* c)
+{
+    if (!c) return TWDIC_EINVAL;
+    (void)hipSetDevice(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    void* bufs[] = {c->d_ref, c->d_tar, c->d_rx, c->d_ry, c->d_tx, c->d_ty, c->d_txy, c->d_lut,
+                    c->d_fft_tw, c->d_refstate};
+    for (void* b : bufs)
+        if (b) (void)hipFree(b);
+    if (c->h_stage) (void)hipHostFree(c->h_stage);
+    for (auto& ev : c->ev)
+        if (ev) (void)hipEventDestroy(ev);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+    return TWDIC_OK;
+}
+
+const char* twdic_last_error(const twdic_ctx* c)
+{
+    return c ? c->err.c_str() : "null context";
+}
+
+twdic_status twdic_sync(twdic_ctx* c)
+{
+    if (!c) return TWDIC_EINVAL;
+    TW_HIP(c, hipStreamSynchronize(c->stream));
+    return TWDIC_OK;
+}
+
+void* twdic_stream(twdic_ctx* c)
+{
+    return c ? (void*)c->stream : nullptr;
+}
+
+twdic_status twdic_set_reference(twdic_ctx* c, const uint8_t* img, int stride, int on_device)
+{
+    if (!c) return TWDIC_EINVAL;
+    twdic_status st = upload(c, c->d_ref, img, stride, on_device);
+    if (st != TWDIC_OK) return st;
+    c->ref_loaded = true;
+    c->ref_grads_valid = false;
+    ++c->ref_version;
+    return TWDIC_OK;
+}
+
+twdic_status twdic_set_target(twdic_ctx* c, const uint8_t* img, int stride, int on_device)
+{
+    if (!c) return TWDIC_EINVAL;
+    twdic_status st = upload(c, c->d_tar, img, stride, on_device);
+    if (st != TWDIC_OK) return st;
+    c->tar_loaded = true;
+    c->tar_pre_valid = false;
+    return TWDIC_OK;
+}
+
+twdic_status twdic_poi_grid(twdic_ctx* c, int32_t* d_poi)
+{
+    if (!c || !d_poi) return TWDIC_EINVAL;
+    const twdic_config& g = c->cfg;
+    TW_HIP(c, launch_poi_grid(d_poi, g.roi_x, g.roi_y, c->nx, c->ny, g.mx, g.my, g.sx, g.sy, g.gx, g.gy, c->stream));
+    return TWDIC_OK;
+}
+
+twdic_status twdic_fftcc(twdic_ctx* c, const int32_t* d_poi, int n, float* d_u, float* d_v, float* d_zncc,
+                         uint8_t* d_status)
+{
+    if (!c || n < 0 || (n > 0 && (!d_poi || !d_u || !d_v || !d_zncc))) return fail(c, TWDIC_EINVAL, "twdic_fftcc: bad arguments");
+    if (!c->ref_loaded || !c->tar_loaded) return fail(c, TWDIC_EINVAL, "twdic_fftcc: images not set");
+    FftccParams p{};
+    p.ref = c->d_ref; p.tar = c->d_tar; p.stride = c->dstride;
+    p.img_w = c->cfg.img_w; p.img_h = c->cfg.img_h;
+    p.poi = d_poi; p.n = n; p.N = c->cfg.fft_n; p.tw = c->d_fft_tw;
+    p.tol = 1e-4f;
+    p.u = d_u; p.v = d_v; p.zncc = d_zncc; p.status = d_status;
+    TW_HIP(c, launch_fftcc(p, c->stream));
+    return TWDIC_OK;
+}
+
+twdic_status twdic_precompute_target(twdic_ctx* c)
+{
+    if (!c) return TWDIC_EINVAL;
+    if (!c->tar_loaded) return fail(c, TWDIC_EINVAL, "target image not set");
+    twdic_status st = ensure_ref_grads(c);
+    if (st != TWDIC_OK) return st;
+    const twdic_config& g = c->cfg;
+    TW_HIP(c, launch_gradients(c->d_tar, c->dstride, g.roi_x, g.roi_y, g.roi_w, g.roi_h, c->d_tx, c->d_ty,
+                               c->d_txy, c->stream));
+    TW_HIP(c, launch_bicubic_lut(c->d_tar, c->dstride, g.roi_x, g.roi_y, g.roi_w, g.roi_h, c->d_lut, c->stream));
+    c->tar_pre_valid = true;
+    return TWDIC_OK;
+}
+
+twdic_status twdic_icgn(twdic_ctx* c, const int32_t* d_poi, int n, float* d_u, float* d_v, float* d_p6,
+                        int32_t* d_iters, uint8_t* d_flags, float* d_dpnorm)
+{
+    if (!c || n < 0 || (n > 0 && (!d_poi || !d_u || !d_v))) return fail(c, TWDIC_EINVAL, "twdic_icgn: bad arguments");
+    if (!c->ref_loaded || !c->tar_loaded) return fail(c, TWDIC_EINVAL, "twdic_icgn: images not set");
+    if (n == 0) return TWDIC_OK;
+    const twdic_config& g = c->cfg;
+    twdic_status st = ensure_ref_grads(c);
+    if (st != TWDIC_OK) return st;
+    if (g.interp == TWDIC_INTERP_LUT && !c->tar_pre_valid) {
+        // per-frame target LUT (cuBicubicCoefficients, TW_paDIC_cuICGN2D.cu:844-851)
+        TW_HIP(c, launch_bicubic_lut(c->d_tar, c->dstride, g.roi_x, g.roi_y, g.roi_w, g.roi_h, c->d_lut, c->stream));
+        c->tar_pre_valid = true;
+    }
+    if ((size_t)n > c->refstate_cap) {
+        if (c->d_refstate) {
+            TW_HIP(c, hipStreamSynchronize(c->stream));
+            TW_HIP(c, hipFree(c->d_refstate));
+            c->d_refstate = nullptr;
+        }
+        TW_HIP(c, hipMalloc((void**)&c->d_refstate, sizeof(double) * kRefStateStride * (size_t)n));
+        c->refstate_cap = (size_t)n;
+        c->refstate_version = ~0ull;
+    }
+    IcgnParams p = icgn_params(c, d_poi, n, d_u, d_v, d_p6, d_iters, d_flags, d_dpnorm);
+    if (c->refstate_version != c->ref_version || c->refstate_poi != d_poi || c->refstate_n != n) {
+        // per-reference state (RefAllSubetsNorm_Kernel + InverseHessian_Kernel,
+        // TW_paDIC_cuICGN2D.cu:883-903), rebuilt when the reference or the POI set changed
+        TW_HIP(c, launch_ref_state(p, c->stream));
+        c->refstate_version = c->ref_version;
+        c->refstate_poi = d_poi;
+        c->refstate_n = n;
+    }
+    TW_HIP(c, launch_icgn(p, g.interp, c->stream));
+    return TWDIC_OK;
+}
+
+twdic_status twdic_frame(twdic_ctx* c, const int32_t* d_poi, int n, float* d_u, float* d_v, float* d_zncc,
+                         float* d_p6, int32_t* d_iters, uint8_t* d_flags)
+{
+    twdic_status st = twdic_fftcc(c, d_poi, n, d_u, d_v, d_zncc, nullptr);
+    if (st != TWDIC_OK) return st;
+    return twdic_icgn(c, d_poi, n, d_u, d_v, d_p6, d_iters, d_flags, nullptr);
+}
+
+twdic_status twdic_get_buffer(twdic_ctx* c, int which, void** d_ptr, size_t* bytes)
+{
+    if (!c || !d_ptr) return TWDIC_EINVAL;
+    const size_t roi = (size_t)c->cfg.roi_w * c->cfg.roi_h;
+    const size_t img = (size_t)c->dstride * c->cfg.img_h;
+    void* p = nullptr;
+    size_t b = roi * 4;
+    switch (which) {
+    case TWDIC_BUF_RX: p = c->d_rx; break;
+    case TWDIC_BUF_RY: p = c->d_ry; break;
+    case TWDIC_BUF_TX: p = c->d_tx; break;
+    case TWDIC_BUF_TY: p = c->d_ty; break;
+    case TWDIC_BUF_TXY: p = c->d_txy; break;
+    case TWDIC_BUF_LUT: p = c->d_lut; b = roi * 64; break;
+    case TWDIC_BUF_REF: p = c->d_ref; b = img; break;
+    case TWDIC_BUF_TAR: p = c->d_tar; b = img; break;
+    default: return fail(c, TWDIC_EINVAL, "unknown buffer");
+    }
+    if (which == TWDIC_BUF_RX || which == TWDIC_BUF_RY) {
+        twdic_status st = ensure_ref_grads(c);
+        if (st != TWDIC_OK) return st;
+    }
+    *d_ptr = p;
+    if (bytes) *bytes = b;
+    return TWDIC_OK;
+}
+
+twdic_status twdic_malloc(twdic_ctx* c, void** d_ptr, size_t bytes)
+{
+    if (!c || !d_ptr) return TWDIC_EINVAL;
+    hipError_t e = hipMalloc(d_ptr, bytes ? bytes : 1);
+    if (e == hipErrorOutOfMemory) return fail(c, TWDIC_ENOMEM, "hipMalloc: out of memory");
+    TW_HIP(c, e);
+    return TWDIC_OK;
+}
+
+twdic_status twdic_free(twdic_ctx* c, void* d_ptr)
+{
+    if (!c) return TWDIC_EINVAL;
+    if (!d_ptr) return TWDIC_OK;
+    TW_HIP(c, hipStreamSynchronize(c->stream));
+    TW_HIP(c, hipFree(d_ptr));
+    return TWDIC_OK;
+}
+
+twdic_status twdic_memcpy(twdic_ctx* c, void* dst, const void* src, size_t bytes, int kind)
+{
+    if (!c || (bytes && (!dst || !src))) return TWDIC_EINVAL;
+    if (!bytes) return TWDIC_OK;
+    hipMemcpyKind k = kind == 0 ? hipMemcpyHostToDevice : kind == 1 ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice;
+    if (kind < 0 || kind > 2) return fail(c, TWDIC_EINVAL, "memcpy kind");
+    TW_HIP(c, hipMemcpyAsync(dst, src, bytes, k, c->stream));
+    if (kind != 2) TW_HIP(c, hipStreamSynchronize(c->stream));
+    return TWDIC_OK;
+}
+
+twdic_status twdic_memset(twdic_ctx* c, void* d_ptr, int value, size_t bytes)
+{
+    if (!c || (bytes && !d_ptr)) return TWDIC_EINVAL;
+    if (!bytes) return TWDIC_OK;
+    TW_HIP(c, hipMemsetAsync(d_ptr, value, bytes, c->stream));
+    return TWDIC_OK;
+}
+
+twdic_status twdic_event_record(twdic_ctx* c, int slot)
+{
+    if (!c || slot < 0 || slot >= 15) return TWDIC_EINVAL;
+    TW_HIP(c, hipEventRecord(c->ev[slot], c->stream));
+    return TWDIC_OK;
+}
+
+twdic_status twdic_event_elapsed(twdic_ctx* c, int a, int b, float* ms)
+{
+    if (!c || !ms || a < 0 || b < 0 || a >= 15 || b >= 15) return TWDIC_EINVAL;
+    TW_HIP(c, hipEventSynchronize(c->ev[b]));
+    TW_HIP(c, hipEventElapsedTime(ms, c->ev[a], c->ev[b]));
+    return TWDIC_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,125 @@
+// twdic_internal.h -- shared declarations of the HIP implementation of the
+// paDIC hot path (context, kernel parameter blocks, launch wrappers).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <string>
+
+#include "twdic.h"
+
+namespace twdic {
+
+constexpr int kWave = 64;             // CDNA wavefront width
+constexpr int kMaxFftN = 64;          // largest FFT-CC window edge held in LDS
+constexpr int kRefStateStride = 36;   // doubles of per-reference state per POI
+
+// Per-reference, per-POI state (doubles, kRefStateStride per POI):
+//   [0..20]  FP64 Cholesky factor L of the 6x6 Hessian, packed lower, row-major
+//   [21..26] A_k = sum SD_k               (steepest-descent sums)
+//   [27..32] BR_k = sum SD_k R            (raw reference projections)
+//   [33]     |R - mean R|_2               (R_s)
+//   [34]     reference flag: 0 ok, TWDIC_ICGN_DARK_SUBSET, TWDIC_ICGN_SINGULAR_HESSIAN,
+//            TWDIC_ICGN_OUT_OF_ROI (subset leaves the ROI)
+//   [35]     mean R
+enum { RS_L = 0, RS_A = 21, RS_B = 27, RS_NORM = 33, RS_FLAG = 34, RS_MEAN = 35 };
+
+// Mean and L2 spread of n samples from their exact-or-FP64 sums s1 = sum x,
+// s2 = sum x^2.  Shared by the reference-state and iteration kernels so that
+// identical subsets give bit-identical statistics.
+__device__ __forceinline__ void mean_spread(double s1, double s2, double n, double& mean, double& spread)
+{
+    mean = s1 / n;
+    double var = __fma_rn(-s1, mean, s2);
+    spread = sqrt(var > 0.0 ? var : 0.0);
+}
+
+// sum SD_k (x - mean) from sum SD_k x and sum SD_k.
+__device__ __forceinline__ double centred(double sdx, double mean, double a)
+{
+    return __fma_rn(-mean, a, sdx);
+}
+
+struct FftccParams {
+    const uint8_t* ref;
+    const uint8_t* tar;
+    int stride;
+    int img_w, img_h;
+    const int32_t* poi;
+    int n;
+    int N;
+    const float2* tw;     // N twiddles exp(-2 pi i e / N)
+    float tol;            // near-tie tolerance in ZNCC units
+    float* u;
+    float* v;
+    float* zncc;
+    uint8_t* status;
+};
+
+struct IcgnParams {
+    const uint8_t* ref;
+    const uint8_t* tar;
+    int stride;
+    int sx0, sy0, rw, rh;     // ROI
+    int Sx, Sy, Wsub, Hsub, s;
+    int max_iter;
+    float delta_p;
+    const float* rx;          // reference gradients, ROI-relative
+    const float* ry;
+    const float* lut;         // target LUT [rh][rw][16]
+    double* refstate;         // kRefStateStride doubles per POI
+    const int32_t* poi;
+    int n;
+    float* u;
+    float* v;
+    float* p6;
+    int32_t* iters;
+    uint8_t* flags;
+    float* dpnorm;
+};
+
+// Kernel launch wrappers (return hipError_t of the launch).
+hipError_t launch_gradients(const uint8_t* img, int stride, int sx0, int sy0, int rw, int rh,
+                            float* gx, float* gy, float* gxy, hipStream_t st);
+hipError_t launch_bicubic_lut(const uint8_t* tar, int stride, int sx0, int sy0, int rw, int rh,
+                              float* lut, hipStream_t st);
+hipError_t launch_poi_grid(int32_t* poi, int start_x, int start_y, int nx, int ny, int mx, int my,
+                           int sx, int sy, int gx, int gy, hipStream_t st);
+hipError_t launch_fftcc(const FftccParams& p, hipStream_t st);
+hipError_t launch_ref_state(const IcgnParams& p, hipStream_t st);
+hipError_t launch_icgn(const IcgnParams& p, int interp, hipStream_t st);
+
+}  // namespace twdic
+
+struct twdic_ctx {
+    twdic_config cfg{};
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int nx = 0, ny = 0;
+    int dstride = 0;                  // device image row stride (bytes)
+    uint8_t* d_ref = nullptr;
+    uint8_t* d_tar = nullptr;
+    float* d_rx = nullptr;
+    float* d_ry = nullptr;
+    float* d_tx = nullptr;
+    float* d_ty = nullptr;
+    float* d_txy = nullptr;
+    float* d_lut = nullptr;
+    float2* d_fft_tw = nullptr;
+    bool ref_loaded = false, tar_loaded = false;
+    bool ref_grads_valid = false;     // Rx/Ry match d_ref
+    bool tar_pre_valid = false;       // Tx/Ty/Txy/LUT match d_tar
+    uint64_t ref_version = 0;
+    // per-reference POI state cache key
+    double* d_refstate = nullptr;
+    size_t refstate_cap = 0;          // POIs
+    uint64_t refstate_version = ~0ull;
+    const int32_t* refstate_poi = nullptr;
+    int refstate_n = -1;
+    uint8_t* h_stage = nullptr;       // pinned staging for host images
+    size_t h_stage_bytes = 0;
+    hipEvent_t ev[16] = {};
+    std::string err;
+};
