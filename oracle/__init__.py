@@ -51,6 +51,8 @@ def lib() -> C.CDLL:
             "orc_potrf6_f64": ([vp], i),
             "orc_potrs6_f64": ([vp, vp], None),
             "orc_lut_eval_f32": ([vp, f, f], f),
+            "orc_warp_update_f32": ([vp, vp], i),
+            "orc_warp_update_f64": ([vp, vp], i),
         }
         for name, (a, r) in sig.items():
             fn = getattr(L, name)
@@ -172,6 +174,15 @@ def cholesky_solve(H: np.ndarray, b: np.ndarray, precision="f64"):
     info = potrf(_p(a))
     potrs(_p(a), _p(rhs))
     return info, np.tril(a), rhs
+
+
+def warp_update(p, dp, precision="f64"):
+    dt = np.float32 if precision == "f32" else np.float64
+    a = np.ascontiguousarray(p, dt).copy()
+    d = np.ascontiguousarray(dp, dt)
+    fn = lib().orc_warp_update_f32 if precision == "f32" else lib().orc_warp_update_f64
+    singular = fn(_p(a), _p(d))
+    return a, bool(singular)
 
 
 def lut_eval(cell16: np.ndarray, tx: float, ty: float) -> float:

@@ -132,6 +132,11 @@ int orc_potrf6_f32(float* a);
 void orc_potrs6_f32(const float* l, float* b);
 int orc_potrf6_f64(double* a);
 void orc_potrs6_f64(const double* l, double* b);
+/* One inverse-compositional update W(p) <- W(p) o W(dp)^-1 with the closed
+ * form of TW_paDIC_ICGN2D_CPU.cpp:713-745; p is updated in place.  Returns 1
+ * when |det W(dp)| <= epsilon (SingularWarp; p untouched), else 0. */
+int orc_warp_update_f32(float* p, const float* dp);
+int orc_warp_update_f64(double* p, const double* dp);
 /* Bicubic evaluation of one LUT cell the way ICGN2D_Compute does
  * (TW_paDIC_ICGN2D_CPU.cpp:775-784). */
 float orc_lut_eval_f32(const float* cell16, float tx, float ty);

@@ -300,6 +300,15 @@ static void SFX(warp_update)(REAL* P, const REAL* dp, REAL W[2][3], REAL den)
     P[5] = W[1][1] - 1;
 }
 
+int SFX(orc_warp_update)(REAL* p, const REAL* dp)
+{
+    REAL W[2][3];
+    REAL den = (1 + dp[1]) * (1 + dp[5]) - dp[2] * dp[4];
+    if (RABS(den) <= REAL_EPS) return 1;
+    SFX(warp_update)(p, dp, W, den);
+    return 0;
+}
+
 static double SFX(conv_norm)(const REAL* dp, int Sx, int Sy)
 {
     /* :751 -- pow(real_t, int) promotes to double under ISO C++11. */
@@ -320,6 +329,10 @@ static int SFX(icgn_poi)(const SFX(icgn_ctx)* c, REAL* fU, REAL* fV, int32_t* it
     *dpn = NAN;
     P[0] = *fU; P[3] = *fV;
     memcpy(Pout, P, sizeof(P));
+    /* Extension: a subset that leaves the ROI would make the reference read
+     * outside its gradient arrays (:543-544); report OutofROI instead. */
+    if (px - Sx < c->sx0 || py - Sy < c->sy0 || px + Sx >= c->sx0 + c->rw || py + Sy >= c->sy0 + c->rh)
+        return ORC_OUT_OF_ROI;
     for (int l = 0; l < H; ++l)                                  /* :516-564 */
         for (int m = 0; m < W; ++m) {
             int idY = py - Sy + l, idX = px - Sx + m;

@@ -1,33 +1,32 @@
-// fftcc.hip -- FFT-CC integer-pixel search for gfx950, one workgroup per POI.
+// fftcc.hip -- FFT-CC integer-pixel search for gfx950.
 //
 // Semantics follow Fftcc2D_CPU::ComputeFFTCC (TW_paDIC_FFTCC2D_CPU.cpp:
 // 222-326) with the window edge N as a parameter (N = 2S is the reference):
 //   window rows POIy - N/2 .. POIy + N/2 - 1, same for columns;
 //   C[k] = sum_x R~(x) T~(x+k) (circular), peak = argmax C, lowest row-major
 //   index on ties; u = k % N, v = k / N, wrapped by -N when > N/2;
-//   ZNCC = C[peak] / (N^2 sqrt(sum R~^2 sum T~^2)).
+//   ZNCC = C[peak] / sqrt(sum R~^2 sum T~^2).
 //
-// MI355X design:
-//   * Both windows are packed into one complex array Z = R~ + i T~ held in
-//     LDS, so one forward 2-D transform yields both spectra
-//     (F_R[k] = (Z[k] + conj Z[-k]) / 2, F_T[k] = (Z[k] - conj Z[-k]) / 2i).
-//   * The cross-power spectrum conj(F_R) F_T = Im(Z[k] Z[-k]) / 2
-//     - i (|Z[k]|^2 - |Z[-k]|^2) / 4 is formed in place, then inverse
-//     transformed; nothing round-trips HBM.
-//   * Exact peak: the zero-mean correlation differs from the integer
-//     correlation I[k] = sum_x R(x) T(x+k) by a constant, so the true peak is
-//     an integer argmax.  The FP32 surface gives the maximum; every shift
-//     within `tol` (ZNCC units) of it is re-evaluated exactly in integers
-//     from the uint8 windows kept in LDS.  With a single candidate (the
-//     usual case) the FP32 argmax is provably exact and no integer work runs.
+// Exactness: the zero-mean correlation differs from the integer correlation
+// I[k] = sum_x R(x) T(x+k) by a constant, so the true peak is an integer
+// argmax.  The FP32 transforms give the maximum; when another shift lies
+// within `tol` (ZNCC units) of it, all such shifts are re-evaluated exactly in
+// integers (lowest index wins ties).  With a single candidate the FP32 argmax
+// is provably the exact one.  The reported ZNCC is evaluated in FP64 from the
+// exact integer sums at the chosen peak, so every output is a function of the
+// two windows alone: results do not depend on batching or pairing.
 //
-// Two transform engines share the rest of the kernel:
-//   * radix-4/2 register FFT for N = 32 and N = 64 (fast path): each of
-//     256 threads owns N*N/256 points, 64-point transforms run as 16 x 4
-//     (16-point DFT in registers, 4-point DFT across a lane quad via DPP
-//     shuffles); transposes go through LDS;
-//   * direct DFT by rows/columns for any other even N <= 64 (reference
-//     parity mode N = 2S = 20, 30, 40, 60).
+// Two transform engines:
+//   * k_fftcc_fft<P> (N = 4P = 32 or 64): one workgroup of 4N threads owns a
+//     POI pair.  Each window pair is packed into Z = R^ + i T^ (R^, T^
+//     zero-mean, unit norm), so one forward 2-D FFT yields both spectra; the
+//     cross-power spectra X_A, X_B are Hermitian, so a single inverse FFT of
+//     X_A + i X_B returns both correlation surfaces (real and imaginary
+//     parts): 3 complex transforms per 2 POIs.  A 4N-point line transform
+//     runs as a P-point radix-4 DFT in registers and a 4-point DFT across a
+//     lane quad with DPP quad permutes; transposes go through LDS.
+//   * k_fftcc_dft: direct DFT along rows/columns for any other even N <= 64
+//     (the reference's own window N = 2S = 20, 30, 40, 60).
 #include "twdic_internal.h"
 
 #include <cfloat>
@@ -36,8 +35,7 @@ namespace twdic {
 
 namespace {
 
-constexpr int kThreads = 256;
-constexpr int kPitch = kMaxFftN + 1;   // LDS row pitch in float2, avoids bank conflicts
+constexpr float kTwoPi = 6.283185307179586f;
 
 struct Top2 {
     float best;
@@ -69,15 +67,14 @@ __device__ __forceinline__ Top2 wave_top2(Top2 t)
     return t;
 }
 
-template <typename T>
-__device__ __forceinline__ T wave_sum(T v)
+__device__ __forceinline__ int wave_sum_i(int v)
 {
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
     return v;
 }
 
-__device__ __forceinline__ int wave_min(int v)
+__device__ __forceinline__ int wave_min_i(int v)
 {
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o));
@@ -89,24 +86,429 @@ __device__ __forceinline__ float2 cmul(float2 a, float2 b)
     return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
 }
 
-// Shared-memory layout of one workgroup.
-struct FftccSmem {
-    float2 z[kMaxFftN * kPitch];         // 33,280 B
-    uint8_t wr[kMaxFftN * kMaxFftN];     // reference window (exact refinement)
-    uint8_t wt[kMaxFftN * kMaxFftN];     // target window
+__device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+
+// i*SGN*d
+template <int SGN>
+__device__ __forceinline__ float2 rot90(float2 d)
+{
+    return SGN > 0 ? make_float2(-d.y, d.x) : make_float2(d.y, -d.x);
+}
+
+// Add one value to a running top-2 whose indices arrive in increasing order
+// (equal values keep the earlier, lower index).  Branch-free.
+__device__ __forceinline__ void top2_push(Top2& t, float c, int idx)
+{
+    t.second = fmaxf(t.second, fminf(t.best, c));
+    t.idx = c > t.best ? idx : t.idx;
+    t.best = fmaxf(t.best, c);
+}
+
+// Wrap a row-major peak index to the reference's (u, v) (:316-322).
+__device__ __forceinline__ void wrap_peak(int k, int N, int& u, int& v)
+{
+    u = k % N;
+    v = k / N;
+    if (u > N / 2) u -= N;
+    if (v > N / 2) v -= N;
+}
+
+// ----------------------------------------------------------- register DFTs
+// 4-point DFT, natural order, X[k] = sum a[n] exp(i SGN 2 pi n k / 4).
+template <int SGN>
+__device__ __forceinline__ void dft4(float2& a0, float2& a1, float2& a2, float2& a3)
+{
+    float2 t0 = cadd(a0, a2), t1 = csub(a0, a2), t2 = cadd(a1, a3), t3 = rot90<SGN>(csub(a1, a3));
+    a0 = cadd(t0, t2);
+    a2 = csub(t0, t2);
+    a1 = cadd(t1, t3);
+    a3 = csub(t1, t3);
+}
+
+template <int SGN>
+__device__ __forceinline__ float2 twc(float2 a, int e, int n)   // a * exp(i SGN 2 pi e / n), e,n constant
+{
+    const float c = __builtin_cosf(kTwoPi * (float)e / (float)n);
+    const float s = (float)SGN * __builtin_sinf(kTwoPi * (float)e / (float)n);
+    return make_float2(a.x * c - a.y * s, a.x * s + a.y * c);
+}
+
+// P-point DFT in registers, natural order in and out (P = 16: 4 x 4; P = 8: 4 x 2).
+template <int P, int SGN>
+__device__ __forceinline__ void dft_reg(float2 (&x)[P])
+{
+    if constexpr (P == 16) {
+#pragma unroll
+        for (int n2 = 0; n2 < 4; ++n2) dft4<SGN>(x[n2], x[4 + n2], x[8 + n2], x[12 + n2]);
+        // x[4*k1 + n2] now holds Y[n2][k1]; twiddle by W16^(n2 k1)
+#pragma unroll
+        for (int n2 = 1; n2 < 4; ++n2)
+#pragma unroll
+            for (int k1 = 1; k1 < 4; ++k1) x[4 * k1 + n2] = twc<SGN>(x[4 * k1 + n2], n2 * k1, 16);
+        float2 y[16];
+#pragma unroll
+        for (int k1 = 0; k1 < 4; ++k1) {
+            float2 a0 = x[4 * k1], a1 = x[4 * k1 + 1], a2 = x[4 * k1 + 2], a3 = x[4 * k1 + 3];
+            dft4<SGN>(a0, a1, a2, a3);
+            y[k1] = a0; y[k1 + 4] = a1; y[k1 + 8] = a2; y[k1 + 12] = a3;
+        }
+#pragma unroll
+        for (int i = 0; i < 16; ++i) x[i] = y[i];
+    } else {
+        static_assert(P == 8, "P must be 8 or 16");
+        // n = 2 n1 + n2: 4-point DFTs over n1 for n2 = 0, 1
+        dft4<SGN>(x[0], x[2], x[4], x[6]);
+        dft4<SGN>(x[1], x[3], x[5], x[7]);
+        float2 y[8];
+#pragma unroll
+        for (int k1 = 0; k1 < 4; ++k1) {
+            float2 a = x[2 * k1], b = twc<SGN>(x[2 * k1 + 1], k1, 8);
+            y[k1] = cadd(a, b);
+            y[k1 + 4] = csub(a, b);
+        }
+#pragma unroll
+        for (int i = 0; i < 8; ++i) x[i] = y[i];
+    }
+}
+
+// ------------------------------------------------- 4-point DFT across a quad
+// DPP quad permutes: xor 1 = quad_perm [1,0,3,2] (0xB1), xor 2 = [2,3,0,1] (0x4E).
+__device__ __forceinline__ float dpp_xor1(float v)
+{
+    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0xB1, 0xF, 0xF, true));
+}
+__device__ __forceinline__ float dpp_xor2(float v)
+{
+    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x4E, 0xF, 0xF, true));
+}
+
+// DIT form: lane q holds index q; returns index brev2(q).
+template <int SGN>
+__device__ __forceinline__ float2 quad_dit(float2 v, int q)
+{
+    const float s2 = (q < 2) ? 1.f : -1.f, s1 = (q & 1) ? -1.f : 1.f;
+    float2 a = make_float2(fmaf(s2, v.x, dpp_xor2(v.x)), fmaf(s2, v.y, dpp_xor2(v.y)));
+    if (q == 3) a = rot90<SGN>(a);
+    return make_float2(fmaf(s1, a.x, dpp_xor1(a.x)), fmaf(s1, a.y, dpp_xor1(a.y)));
+}
+
+// DIF form: lane q holds index brev2(q); returns index q.
+template <int SGN>
+__device__ __forceinline__ float2 quad_dif(float2 v, int q)
+{
+    const float s2 = (q < 2) ? 1.f : -1.f, s1 = (q & 1) ? -1.f : 1.f;
+    float2 a = make_float2(fmaf(s1, v.x, dpp_xor1(v.x)), fmaf(s1, v.y, dpp_xor1(v.y)));
+    if (q == 3) a = rot90<SGN>(a);
+    return make_float2(fmaf(s2, a.x, dpp_xor2(a.x)), fmaf(s2, a.y, dpp_xor2(a.y)));
+}
+
+// 4P-point line transforms over a lane quad (tw: LDS table exp(-2 pi i e / 4P)).
+//   dif: lane q holds x[j + P*brev2(q)]  ->  X[4k + q]
+//   dit: lane q holds x[4j + q]          ->  X[k + P*brev2(q)]
+template <int P, int SGN>
+__device__ __forceinline__ void line_dif(float2 (&x)[P], int q, const float2* tw)
+{
+#pragma unroll
+    for (int j = 0; j < P; ++j) x[j] = quad_dif<SGN>(x[j], q);
+#pragma unroll
+    for (int j = 1; j < P; ++j) {
+        float2 w = tw[(q * j) & (4 * P - 1)];
+        if (SGN > 0) w.y = -w.y;
+        x[j] = cmul(x[j], w);
+    }
+    dft_reg<P, SGN>(x);
+}
+
+template <int P, int SGN>
+__device__ __forceinline__ void line_dit(float2 (&x)[P], int q, const float2* tw)
+{
+    dft_reg<P, SGN>(x);
+#pragma unroll
+    for (int k = 1; k < P; ++k) {
+        float2 w = tw[(q * k) & (4 * P - 1)];
+        if (SGN > 0) w.y = -w.y;
+        x[k] = cmul(x[k], w);
+    }
+#pragma unroll
+    for (int k = 0; k < P; ++k) x[k] = quad_dit<SGN>(x[k], q);
+}
+
+}  // namespace
+
+// ===========================================================================
+// Fast engine: N = 4P in {32, 64}; 4N threads per workgroup, one POI pair.
+// Thread t: line = t / 4 (row or column being transformed), q = t % 4.
+// ===========================================================================
+template <int P>
+struct FftSmem {
+    static constexpr int N = 4 * P;
+    // complex plane, element (r, c) at r*(N+1) + c + r/P (row skew spreads
+    // the four P-row blocks of a column read over distinct banks)
+    float2 z[N * (N + 1) + 4];
+    float2 tw[N];
+    int red[N / 16][8];
+    Top2 top[N / 16][2];
+    int imin[N / 16];
+};
+
+template <int P>
+__device__ __forceinline__ int zidx(int r, int c)
+{
+    return r * (4 * P + 1) + c + r / P;
+}
+
+// Exact integer correlation of the thread's R pixels with the LDS T window
+// (bytes, row-major N x N) at shift (ku, kv), reduced over the workgroup.
+template <int P>
+__device__ __forceinline__ int exact_corr(const uint32_t (&rw)[P / 4], const uint8_t* tb, int line, int bq, int ku,
+                                          int kv, FftSmem<P>& sm, int slot)
+{
+    constexpr int N = 4 * P;
+    const int row = ((line + kv) & (N - 1)) * N;
+    int acc = 0;
+#pragma unroll
+    for (int j = 0; j < P; ++j) {
+        const int r = (rw[j >> 2] >> (8 * (j & 3))) & 255;
+        acc += r * (int)tb[row + ((bq * P + j + ku) & (N - 1))];
+    }
+    acc = wave_sum_i(acc);
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    if (lane == 0) sm.red[wave][slot] = acc;
+    __syncthreads();
+    int s = 0;
+#pragma unroll
+    for (int w = 0; w < N / 16; ++w) s += sm.red[w][slot];
+    __syncthreads();
+    return s;
+}
+
+template <int P, int MINW>
+__global__ __launch_bounds__(16 * P, MINW) void k_fftcc_fft(FftccParams p)
+{
+    constexpr int N = 4 * P, NN = N * N, NW = N / 16;   // waves per workgroup
+    __shared__ FftSmem<P> sm;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int line = tid >> 2, q = tid & 3, bq = ((q & 1) << 1) | (q >> 1);
+    const int poi0 = 2 * blockIdx.x;
+
+    if (tid < N) sm.tw[tid] = p.tw[tid];
+
+    // -- 1. load both window pairs straight into registers (DIF layout: row
+    //       `line`, columns bq*P .. bq*P+P-1), exact integer sums ---------------
+    uint32_t rw[2][P / 4], tw8[2][P / 4];
+    int valid[2], sums[2][4];
+    int py[2], px[2];
+#pragma unroll
+    for (int s = 0; s < 2; ++s) {
+        const int poi = poi0 + s;
+        valid[s] = 0;
+        py[s] = px[s] = 0;
+        if (poi < p.n) {
+            py[s] = p.poi[2 * poi];
+            px[s] = p.poi[2 * poi + 1];
+            const int y0 = py[s] - N / 2, x0 = px[s] - N / 2;
+            valid[s] = (y0 >= 0 && x0 >= 0 && y0 + N <= p.img_h && x0 + N <= p.img_w) ? 1 : 2;
+        }
+#pragma unroll
+        for (int k = 0; k < P / 4; ++k) { rw[s][k] = 0u; tw8[s][k] = 0u; }
+        if (valid[s] == 1) {
+            const size_t o = (size_t)(py[s] - N / 2 + line) * p.stride + (px[s] - N / 2 + bq * P);
+            __builtin_memcpy(rw[s], p.ref + o, P);
+            __builtin_memcpy(tw8[s], p.tar + o, P);
+        }
+        int a = 0, b = 0, aa = 0, bb = 0;
+#pragma unroll
+        for (int k = 0; k < P; ++k) {
+            const int r = (rw[s][k >> 2] >> (8 * (k & 3))) & 255, t = (tw8[s][k >> 2] >> (8 * (k & 3))) & 255;
+            a += r; b += t; aa += r * r; bb += t * t;
+        }
+        sums[s][0] = wave_sum_i(a); sums[s][1] = wave_sum_i(b);
+        sums[s][2] = wave_sum_i(aa); sums[s][3] = wave_sum_i(bb);
+        if (lane == 0) {
+#pragma unroll
+            for (int k = 0; k < 4; ++k) sm.red[wave][4 * s + k] = sums[s][k];
+        }
+    }
+    __syncthreads();
+    double nr[2], nt[2];
+    float mr[2], mt[2], gr[2], gt[2];
+#pragma unroll
+    for (int s = 0; s < 2; ++s) {
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            int acc = 0;
+#pragma unroll
+            for (int w = 0; w < NW; ++w) acc += sm.red[w][4 * s + k];
+            sums[s][k] = acc;
+        }
+        const long long sr = sums[s][0], st = sums[s][1], srr = sums[s][2], stt = sums[s][3];
+        // flat window (TW_paDIC_FFTCC2D_CPU.cpp:271-275): exact integer test
+        if (valid[s] == 1 && (srr * NN == sr * sr || stt * NN == st * st)) valid[s] = 3;
+        nr[s] = (double)srr - (double)sr * (double)sr / (double)NN;
+        nt[s] = (double)stt - (double)st * (double)st / (double)NN;
+        mr[s] = (float)sr / (float)NN;          // (:253-254)
+        mt[s] = (float)st / (float)NN;
+        gr[s] = valid[s] == 1 ? (float)(1.0 / sqrt(nr[s])) : 0.f;
+        gt[s] = valid[s] == 1 ? (float)(1.0 / sqrt(nt[s])) : 0.f;
+    }
+    __syncthreads();
+
+    // -- 2. forward transforms and cross-power spectra ---------------------------
+    float2 X[2][P];
+#pragma unroll
+    for (int s = 0; s < 2; ++s) {
+        float2 x[P];
+#pragma unroll
+        for (int j = 0; j < P; ++j) {
+            const int r = (rw[s][j >> 2] >> (8 * (j & 3))) & 255, t = (tw8[s][j >> 2] >> (8 * (j & 3))) & 255;
+            x[j] = make_float2(((float)r - mr[s]) * gr[s], ((float)t - mt[s]) * gt[s]);
+        }
+        line_dif<P, -1>(x, q, sm.tw);                       // rows: lane holds kx = 4j + q
+#pragma unroll
+        for (int j = 0; j < P; ++j) sm.z[zidx<P>(line, 4 * j + q)] = x[j];
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < P; ++j) x[j] = sm.z[zidx<P>(bq * P + j, line)];
+        __syncthreads();
+        line_dif<P, -1>(x, q, sm.tw);                       // columns: lane holds ky = 4j + q, kx = line
+#pragma unroll
+        for (int j = 0; j < P; ++j) sm.z[zidx<P>(4 * j + q, line)] = x[j];
+        __syncthreads();
+        // conj(F_R) F_T = Im(Z[k] Z[-k]) / 2 - i (|Z[k]|^2 - |Z[-k]|^2) / 4
+        const int mkx = (N - line) & (N - 1);
+#pragma unroll
+        for (int j = 0; j < P; ++j) {
+            const int mky = (N - (4 * j + q)) & (N - 1);
+            const float2 a = x[j], b = sm.z[zidx<P>(mky, mkx)];
+            const float im_ab = a.x * b.y + a.y * b.x;
+            const float d = (a.x * a.x + a.y * a.y) - (b.x * b.x + b.y * b.y);
+            X[s][j] = make_float2(0.5f * im_ab, -0.25f * d);
+        }
+        __syncthreads();
+    }
+
+    // -- 3. one inverse transform for both: IFFT(X_A + i X_B) = c_A + i c_B -------
+    float2 x[P];
+#pragma unroll
+    for (int j = 0; j < P; ++j) x[j] = make_float2(X[0][j].x - X[1][j].y, X[0][j].y + X[1][j].x);
+    line_dit<P, 1>(x, q, sm.tw);                            // columns: lane holds y = j + P*bq
+#pragma unroll
+    for (int j = 0; j < P; ++j) sm.z[zidx<P>(j + P * bq, line)] = x[j];
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < P; ++j) x[j] = sm.z[zidx<P>(line, 4 * j + q)];
+    __syncthreads();
+    line_dit<P, 1>(x, q, sm.tw);                            // rows: lane holds x = j + P*bq of row `line`
+
+    // -- 4. peaks: top-2 per POI with lowest-index tie-break (:300-311) ----------
+    Top2 t[2];
+#pragma unroll
+    for (int s = 0; s < 2; ++s) {
+        t[s] = Top2{-FLT_MAX, 0x7fffffff, -FLT_MAX};
+#pragma unroll
+        for (int j = 0; j < P; ++j) top2_push(t[s], s == 0 ? x[j].x : x[j].y, line * N + j + P * bq);
+        t[s] = wave_top2(t[s]);
+        if (lane == 0) sm.top[wave][s] = t[s];
+    }
+    // T windows to LDS (bytes, row-major) for the exact integer sums; the
+    // complex plane is free once every thread holds its outputs.  The window
+    // words are re-read (L2-resident) rather than kept live in registers.
+    uint8_t* tb = reinterpret_cast<uint8_t*>(sm.z);
+#pragma unroll
+    for (int s = 0; s < 2; ++s) {
+        if (valid[s] == 1) {
+            const size_t o = (size_t)(py[s] - N / 2 + line) * p.stride + (px[s] - N / 2 + bq * P);
+            __builtin_memcpy(rw[s], p.ref + o, P);
+            __builtin_memcpy(tw8[s], p.tar + o, P);
+        }
+#pragma unroll
+        for (int k = 0; k < P / 4; ++k)
+            reinterpret_cast<uint32_t*>(tb + s * NN + line * N + bq * P)[k] = tw8[s][k];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int s = 0; s < 2; ++s) {
+        Top2 r = sm.top[0][s];
+#pragma unroll
+        for (int w = 1; w < NW; ++w) r = top2_merge(r, sm.top[w][s]);
+        t[s] = r;
+    }
+
+    // -- 5. exact peak and ZNCC from integers -------------------------------------
+#pragma unroll
+    for (int s = 0; s < 2; ++s) {
+        if (valid[s] != 1) continue;                         // uniform per workgroup
+        const float thr = t[s].best - p.tol * (float)NN;
+        int peak = t[s].idx, bestI = 0;
+        if (t[s].second < thr) {
+            bestI = exact_corr<P>(rw[s], tb + s * NN, line, bq, peak % N, peak / N, sm, 0);
+        } else {
+            // several shifts within tol of the maximum: evaluate them all exactly,
+            // in increasing index order, keeping the first strict maximum
+            bestI = -1;
+            peak = -1;
+            int last = -1;
+            for (;;) {
+                int cand = 0x7fffffff;
+#pragma unroll
+                for (int j = 0; j < P; ++j) {
+                    const int idx = line * N + j + P * bq;
+                    const float c = s == 0 ? x[j].x : x[j].y;
+                    if (idx > last && c >= thr) cand = min(cand, idx);
+                }
+                cand = wave_min_i(cand);
+                if (lane == 0) sm.imin[wave] = cand;
+                __syncthreads();
+                int m = sm.imin[0];
+#pragma unroll
+                for (int w = 1; w < NW; ++w) m = min(m, sm.imin[w]);
+                __syncthreads();
+                if (m == 0x7fffffff) break;
+                const int I = exact_corr<P>(rw[s], tb + s * NN, line, bq, m % N, m / N, sm, 0);
+                if (I > bestI) { bestI = I; peak = m; }
+                last = m;
+            }
+        }
+        if (tid == 0) {
+            int iu, iv;
+            wrap_peak(peak, N, iu, iv);
+            const double sr = sums[s][0], st = sums[s][1];
+            p.u[poi0 + s] = (float)iu;
+            p.v[poi0 + s] = (float)iv;
+            p.zncc[poi0 + s] = (float)(((double)bestI - sr * st / (double)NN) / sqrt(nr[s] * nt[s]));
+        }
+    }
+    if (tid == 0 && p.status) {
+#pragma unroll
+        for (int s = 0; s < 2; ++s)
+            if (valid[s]) p.status[poi0 + s] = valid[s] == 1 ? TWDIC_FFTCC_OK : valid[s] == 2 ? TWDIC_FFTCC_OUT
+                                                                                                : TWDIC_FFTCC_FLAT;
+    }
+}
+
+// ===========================================================================
+// Direct-DFT engine: any even N <= 64, one POI per 256-thread workgroup.
+// ===========================================================================
+namespace {
+
+constexpr int kDftThreads = 256;
+constexpr int kPitch = kMaxFftN + 1;
+
+struct DftSmem {
+    float2 z[kMaxFftN * kPitch];
+    uint8_t wr[kMaxFftN * kMaxFftN];
+    uint8_t wt[kMaxFftN * kMaxFftN];
     float2 tw[kMaxFftN];
-    long long red[4][4];
+    int red[4][4];
     Top2 top[4];
     int imin[4];
 };
 
-// ---------------------------------------------------------------------------
-// Direct DFT along rows (dir = 0) or columns (dir = 1), in place, with
-// twiddle sign s (conjugated twiddles for the inverse).  Each wavefront owns
-// whole lines; lane k computes output k of its line.  All LDS reads of a
-// line complete before its writes within the same wavefront.
-// ---------------------------------------------------------------------------
-__device__ void dft_lines(FftccSmem& sm, int N, int dir, bool inverse)
+// In-place DFT of every row (dir 0) or column (dir 1); each wavefront owns
+// whole lines and lane k produces output k.  All LDS reads of a line complete
+// before its writes within the wavefront.
+__device__ void dft_lines(DftSmem& sm, int N, int dir, bool inverse)
 {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     for (int line = wave; line < N; line += 4) {
@@ -120,7 +522,7 @@ __device__ void dft_lines(FftccSmem& sm, int N, int dir, bool inverse)
                 acc.x += z.x * w.x - z.y * w.y;
                 acc.y += z.x * w.y + z.y * w.x;
                 e += lane;
-                if (e >= N) e -= N;   // lane < N, so one subtraction suffices
+                if (e >= N) e -= N;   // lane < N: one subtraction suffices
             }
         }
         __builtin_amdgcn_wave_barrier();
@@ -131,137 +533,45 @@ __device__ void dft_lines(FftccSmem& sm, int N, int dir, bool inverse)
     }
 }
 
-// ---------------------------------------------------------------------------
-// 64-point and 32-point transforms spread over a lane quad (4 lanes x 16 or
-// 8 points).  P = points per lane (16 for N = 64, 8 for N = 32).
-//
-//   dif4(x): input lane q holds x[j + P*brev2(q)], j < P (contiguous block);
-//            output lane q holds X[4k + q], k < P.
-//   dit4(x): input lane q holds x[4j + q];
-//            output lane q holds X[k + P*brev2(q)].
-// Both compute X[k] = sum_n x[n] W^{nk}, W = exp(-2 pi i sgn / (4P)).
-// ---------------------------------------------------------------------------
-// P-point DFT in registers: iterative radix-2 DIT over a bit-reversed copy.
-// Twiddles exp(i sgn 2 pi j / m) have compile-time angles after unrolling.
-template <int P>
-__device__ __forceinline__ void dft_small(float2 (&x)[P], float sgn)
+__device__ int block_sum_i(DftSmem& sm, int v)
 {
-    constexpr int LOG = (P == 16) ? 4 : (P == 8) ? 3 : (P == 4) ? 2 : 1;
-    float2 y[P];
-#pragma unroll
-    for (int i = 0; i < P; ++i) {
-        int r = 0;
-#pragma unroll
-        for (int b = 0; b < LOG; ++b) r |= ((i >> b) & 1) << (LOG - 1 - b);
-        y[r] = x[i];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    v = wave_sum_i(v);
+    __syncthreads();
+    if (lane == 0) sm.red[wave][0] = v;
+    __syncthreads();
+    return sm.red[0][0] + sm.red[1][0] + sm.red[2][0] + sm.red[3][0];
+}
+
+__device__ int exact_corr_dft(DftSmem& sm, int N, int ku, int kv)
+{
+    int acc = 0;
+    for (int i = threadIdx.x; i < N * N; i += kDftThreads) {
+        int y = i / N, x = i - y * N;
+        int yy = y + kv; if (yy >= N) yy -= N;
+        int xx = x + ku; if (xx >= N) xx -= N;
+        acc += (int)sm.wr[i] * (int)sm.wt[yy * N + xx];
     }
-#pragma unroll
-    for (int s = 1; s <= LOG; ++s) {
-        const int m = 1 << s, h = m >> 1;
-#pragma unroll
-        for (int k = 0; k < P; k += m) {
-#pragma unroll
-            for (int j = 0; j < h; ++j) {
-                const double th = 2.0 * 3.14159265358979323846 * (double)j / (double)m;
-                const float c = (float)__builtin_cos(th);
-                const float sn = sgn * (float)__builtin_sin(th);
-                float2 t = y[k + j + h];
-                float2 tw = make_float2(t.x * c - t.y * sn, t.x * sn + t.y * c);
-                float2 a = y[k + j];
-                y[k + j] = make_float2(a.x + tw.x, a.y + tw.y);
-                y[k + j + h] = make_float2(a.x - tw.x, a.y - tw.y);
-            }
-        }
-    }
-#pragma unroll
-    for (int i = 0; i < P; ++i) x[i] = y[i];
-}
-
-__device__ __forceinline__ float2 shfl_xor2(float2 v, int m)
-{
-    return make_float2(__shfl_xor(v.x, m), __shfl_xor(v.y, m));
-}
-
-// 4-point DFT across a lane quad.  natural_in: input lane q holds index q,
-// output lane q holds index brev2(q).  Otherwise input is bit-reversed and
-// output natural.  sgn: -1 forward, +1 inverse.
-__device__ __forceinline__ float2 quad_dft_dit(float2 v, int q, float sgn)
-{
-    // stage 1 (xor 2): lanes 0,1 get v + v', lanes 2,3 get v' - v
-    float2 o = shfl_xor2(v, 2);
-    float2 a = (q < 2) ? make_float2(v.x + o.x, v.y + o.y) : make_float2(o.x - v.x, o.y - v.y);
-    // lane 3 twiddle by W4^1 = exp(-i pi/2 sgn'): multiply by (0, sgn)
-    if (q == 3) a = make_float2(-sgn * a.y, sgn * a.x);
-    float2 b = shfl_xor2(a, 1);
-    return (q & 1) ? make_float2(b.x - a.x, b.y - a.y) : make_float2(a.x + b.x, a.y + b.y);
-}
-
-__device__ __forceinline__ float2 quad_dft_dif(float2 v, int q, float sgn)
-{
-    // input lane q holds block brev2(q) = {0,2,1,3}[q]
-    float2 o = shfl_xor2(v, 1);
-    float2 a = (q & 1) ? make_float2(o.x - v.x, o.y - v.y) : make_float2(v.x + o.x, v.y + o.y);
-    if (q == 3) a = make_float2(-sgn * a.y, sgn * a.x);
-    float2 b = shfl_xor2(a, 2);
-    return (q < 2) ? make_float2(a.x + b.x, a.y + b.y) : make_float2(b.x - a.x, b.y - a.y);
-}
-
-// tw: LDS table of exp(-2 pi i e / N); inverse transforms use conj(tw).
-template <int P>
-__device__ __forceinline__ float2 twiddle(const float2* tw, int e, bool inverse)
-{
-    float2 w = tw[e & (4 * P - 1)];
-    if (inverse) w.y = -w.y;
-    return w;
-}
-
-template <int P>
-__device__ __forceinline__ void fft_dif4(float2 (&x)[P], int q, bool inverse, const float2* tw)
-{
-    const float sgn = inverse ? 1.f : -1.f;
-#pragma unroll
-    for (int j = 0; j < P; ++j) x[j] = quad_dft_dif(x[j], q, sgn);
-#pragma unroll
-    for (int j = 1; j < P; ++j) x[j] = cmul(x[j], twiddle<P>(tw, q * j, inverse));
-    dft_small<P>(x, sgn);
-}
-
-template <int P>
-__device__ __forceinline__ void fft_dit4(float2 (&x)[P], int q, bool inverse, const float2* tw)
-{
-    const float sgn = inverse ? 1.f : -1.f;
-    dft_small<P>(x, sgn);
-#pragma unroll
-    for (int k = 1; k < P; ++k) x[k] = cmul(x[k], twiddle<P>(tw, q * k, inverse));
-#pragma unroll
-    for (int k = 0; k < P; ++k) x[k] = quad_dft_dit(x[k], q, sgn);
+    return block_sum_i(sm, acc);
 }
 
 }  // namespace
 
-// ---------------------------------------------------------------------------
-// The FFT-CC kernel.  FAST selects the register radix-4 engine (N = 4*P with
-// P = 16 or 8); otherwise the direct DFT handles any even N <= 64.
-// ---------------------------------------------------------------------------
-template <int P>
-__global__ __launch_bounds__(kThreads) void k_fftcc(FftccParams p)
+__global__ __launch_bounds__(kDftThreads) void k_fftcc_dft(FftccParams p)
 {
-    __shared__ FftccSmem sm;
+    __shared__ DftSmem sm;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int poi = blockIdx.x;
     if (poi >= p.n) return;
-    const int N = (P > 0) ? 4 * P : p.N;
-    const int NN = N * N, half = N >> 1;
+    const int N = p.N, NN = N * N, half = N >> 1;
     const int py = p.poi[2 * poi], px = p.poi[2 * poi + 1];
     const int y0 = py - half, x0 = px - half;
     if (y0 < 0 || x0 < 0 || y0 + N > p.img_h || x0 + N > p.img_w) {
         if (tid == 0 && p.status) p.status[poi] = TWDIC_FFTCC_OUT;
         return;
     }
-
-    // -- 1. gather both windows into LDS, exact integer sums ---------------
-    long long sr = 0, st = 0, srr = 0, stt = 0;
-    for (int i = tid; i < NN; i += kThreads) {
+    int sr = 0, st = 0, srr = 0, stt = 0;
+    for (int i = tid; i < NN; i += kDftThreads) {
         int y = i / N, x = i - y * N;
         size_t o = (size_t)(y0 + y) * p.stride + (x0 + x);
         int a = p.ref[o], b = p.tar[o];
@@ -270,196 +580,117 @@ __global__ __launch_bounds__(kThreads) void k_fftcc(FftccParams p)
         sr += a; st += b; srr += a * a; stt += b * b;
     }
     if (tid < N) sm.tw[tid] = p.tw[tid];
-    sr = wave_sum(sr); st = wave_sum(st); srr = wave_sum(srr); stt = wave_sum(stt);
+    sr = wave_sum_i(sr); st = wave_sum_i(st); srr = wave_sum_i(srr); stt = wave_sum_i(stt);
     if (lane == 0) { sm.red[wave][0] = sr; sm.red[wave][1] = st; sm.red[wave][2] = srr; sm.red[wave][3] = stt; }
     __syncthreads();
-    sr = sm.red[0][0] + sm.red[1][0] + sm.red[2][0] + sm.red[3][0];
-    st = sm.red[0][1] + sm.red[1][1] + sm.red[2][1] + sm.red[3][1];
-    srr = sm.red[0][2] + sm.red[1][2] + sm.red[2][2] + sm.red[3][2];
-    stt = sm.red[0][3] + sm.red[1][3] + sm.red[2][3] + sm.red[3][3];
-    // flat window (TW_paDIC_FFTCC2D_CPU.cpp:271-275): exact integer test
-    if (srr * NN == sr * sr || stt * NN == st * st) {
+    const long long Sr = sm.red[0][0] + sm.red[1][0] + sm.red[2][0] + sm.red[3][0];
+    const long long St = sm.red[0][1] + sm.red[1][1] + sm.red[2][1] + sm.red[3][1];
+    const long long Srr = sm.red[0][2] + sm.red[1][2] + sm.red[2][2] + sm.red[3][2];
+    const long long Stt = sm.red[0][3] + sm.red[1][3] + sm.red[2][3] + sm.red[3][3];
+    if (Srr * NN == Sr * Sr || Stt * NN == St * St) {         // (:271-275)
         if (tid == 0 && p.status) p.status[poi] = TWDIC_FFTCC_FLAT;
         return;
     }
-    // means as the CPU path forms them: exact sum / N^2 in FP32 (:253-254)
-    const float mr = (float)sr / (float)NN, mt = (float)st / (float)NN;
-    // exact norms sum (x - mean)^2 = sum x^2 - (sum x)^2 / N^2
-    const double nr = (double)srr - (double)sr * (double)sr / (double)NN;
-    const double nt = (double)stt - (double)st * (double)st / (double)NN;
-    const float scale = (float)((double)NN * sqrt(nr * nt));
-
-    // -- 2. forward 2-D transform of Z = R~ + i T~ ---------------------------
-    if constexpr (P > 0) {
-        const int q = tid & 3, line = tid >> 2;         // 64 lines x 4 lanes
-        const int bq = ((q & 1) << 1) | (q >> 1);       // brev2(q)
-        float2 x[P];
-        // rows: DIF input = contiguous block bq of row `line`
-        if (line < N) {
-#pragma unroll
-            for (int j = 0; j < P; ++j) {
-                int c = bq * P + j;
-                x[j] = make_float2((float)sm.wr[line * N + c] - mr, (float)sm.wt[line * N + c] - mt);
-            }
-            fft_dif4<P>(x, q, false, sm.tw);
-            // lane holds row spectrum at kx = 4j + q
-#pragma unroll
-            for (int j = 0; j < P; ++j) sm.z[line * kPitch + 4 * j + q] = x[j];
-        }
-        __syncthreads();
-        // columns: DIF input = contiguous block bq of column `line`
-        if (line < N) {
-#pragma unroll
-            for (int j = 0; j < P; ++j) x[j] = sm.z[(bq * P + j) * kPitch + line];
-            fft_dif4<P>(x, q, false, sm.tw);
-        }
-        __syncthreads();
-        if (line < N) {
-#pragma unroll
-            for (int j = 0; j < P; ++j) sm.z[(4 * j + q) * kPitch + line] = x[j];
-        }
-        __syncthreads();
-        // -- 3. cross-power spectrum, kept in registers ----------------------
-        if (line < N) {
-#pragma unroll
-            for (int j = 0; j < P; ++j) {
-                int ky = 4 * j + q, kx = line;
-                int my = (N - ky) & (N - 1), mxx = (N - kx) & (N - 1);
-                float2 a = x[j], b = sm.z[my * kPitch + mxx];
-                float im_ab = a.x * b.y + a.y * b.x;
-                float d = (a.x * a.x + a.y * a.y) - (b.x * b.x + b.y * b.y);
-                x[j] = make_float2(0.5f * im_ab, -0.25f * d);
-            }
-            // -- 4. inverse columns: DIT input lane q holds ky = 4j + q
-            fft_dit4<P>(x, q, true, sm.tw);
-        }
-        __syncthreads();
-        if (line < N) {
-            // lane holds y = j + P*bq of column `line`
-#pragma unroll
-            for (int j = 0; j < P; ++j) sm.z[(j + P * bq) * kPitch + line] = x[j];
-        }
-        __syncthreads();
-        // inverse rows: DIT input lane q holds x = 4j + q of row `line`
-        if (line < N) {
-#pragma unroll
-            for (int j = 0; j < P; ++j) x[j] = sm.z[line * kPitch + 4 * j + q];
-            fft_dit4<P>(x, q, true, sm.tw);
-        }
-        __syncthreads();
-        if (line < N) {
-#pragma unroll
-            for (int j = 0; j < P; ++j) sm.z[line * kPitch + j + P * bq] = x[j];
-        }
-        __syncthreads();
-    } else {
-        for (int i = tid; i < NN; i += kThreads) {
-            int y = i / N, x = i - y * N;
-            sm.z[y * kPitch + x] = make_float2((float)sm.wr[i] - mr, (float)sm.wt[i] - mt);
-        }
-        __syncthreads();
-        dft_lines(sm, N, 0, false);
-        __syncthreads();
-        dft_lines(sm, N, 1, false);
-        __syncthreads();
-        for (int i = tid; i < NN; i += kThreads) {
-            int ky = i / N, kx = i - ky * N;
-            int my = ky ? N - ky : 0, mxx = kx ? N - kx : 0;
-            int j = my * N + mxx;
-            if (i <= j) {
-                float2 a = sm.z[ky * kPitch + kx], b = sm.z[my * kPitch + mxx];
-                float im_ab = a.x * b.y + a.y * b.x;
-                float d = (a.x * a.x + a.y * a.y) - (b.x * b.x + b.y * b.y);
-                float2 xk = make_float2(0.5f * im_ab, -0.25f * d);
-                sm.z[ky * kPitch + kx] = xk;
-                if (j != i) sm.z[my * kPitch + mxx] = make_float2(xk.x, -xk.y);
-            }
-        }
-        __syncthreads();
-        dft_lines(sm, N, 1, true);
-        __syncthreads();
-        dft_lines(sm, N, 0, true);
-        __syncthreads();
-    }
-
-    // -- 5. peak: top-2 with lowest-index tie-break (:300-311) -----------------
-    Top2 t{-FLT_MAX, 0x7fffffff, -FLT_MAX};
-    for (int i = tid; i < NN; i += kThreads) {
+    const double nr = (double)Srr - (double)Sr * (double)Sr / (double)NN;
+    const double nt = (double)Stt - (double)St * (double)St / (double)NN;
+    const float mr = (float)Sr / (float)NN, mt = (float)St / (float)NN;
+    const float gr = (float)(1.0 / sqrt(nr)), gt = (float)(1.0 / sqrt(nt));
+    for (int i = tid; i < NN; i += kDftThreads) {
         int y = i / N, x = i - y * N;
-        float c = sm.z[y * kPitch + x].x;
-        Top2 u{c, i, -FLT_MAX};
-        t = top2_merge(t, u);
+        sm.z[y * kPitch + x] = make_float2(((float)sm.wr[i] - mr) * gr, ((float)sm.wt[i] - mt) * gt);
+    }
+    __syncthreads();
+    dft_lines(sm, N, 0, false);
+    __syncthreads();
+    dft_lines(sm, N, 1, false);
+    __syncthreads();
+    for (int i = tid; i < NN; i += kDftThreads) {
+        int ky = i / N, kx = i - ky * N;
+        int my = ky ? N - ky : 0, mx = kx ? N - kx : 0;
+        int j = my * N + mx;
+        if (i <= j) {          // each Hermitian pair is formed by one thread
+            float2 a = sm.z[ky * kPitch + kx], b = sm.z[my * kPitch + mx];
+            float im_ab = a.x * b.y + a.y * b.x;
+            float d = (a.x * a.x + a.y * a.y) - (b.x * b.x + b.y * b.y);
+            float2 xk = make_float2(0.5f * im_ab, -0.25f * d);
+            sm.z[ky * kPitch + kx] = xk;
+            if (j != i) sm.z[my * kPitch + mx] = make_float2(xk.x, -xk.y);
+        }
+    }
+    __syncthreads();
+    dft_lines(sm, N, 1, true);
+    __syncthreads();
+    dft_lines(sm, N, 0, true);
+    __syncthreads();
+
+    Top2 t{-FLT_MAX, 0x7fffffff, -FLT_MAX};
+    for (int i = tid; i < NN; i += kDftThreads) {
+        int y = i / N, x = i - y * N;
+        t = top2_merge(t, Top2{sm.z[y * kPitch + x].x, i, -FLT_MAX});
     }
     t = wave_top2(t);
     if (lane == 0) sm.top[wave] = t;
     __syncthreads();
     t = top2_merge(top2_merge(sm.top[0], sm.top[1]), top2_merge(sm.top[2], sm.top[3]));
-    int peak = t.idx;
-    float zn = t.best / scale;
-    const float thr = t.best - p.tol * scale;
-    if (t.second >= thr) {
-        // -- 6. exact refinement over all near-maximal shifts -----------------
-        long long bestI = -1;
-        int bestk = -1, last = -1;
+    const float thr = t.best - p.tol * (float)NN;
+    int peak = t.idx, bestI;
+    if (t.second < thr) {
+        bestI = exact_corr_dft(sm, N, peak % N, peak / N);
+    } else {
+        bestI = -1;
+        peak = -1;
+        int last = -1;
         for (;;) {
             int cand = 0x7fffffff;
-            for (int i = tid; i < NN; i += kThreads) {
+            for (int i = tid; i < NN; i += kDftThreads) {
                 int y = i / N, x = i - y * N;
                 if (i > last && sm.z[y * kPitch + x].x >= thr) { cand = i; break; }
             }
-            cand = wave_min(cand);
+            cand = wave_min_i(cand);
             __syncthreads();
             if (lane == 0) sm.imin[wave] = cand;
             __syncthreads();
             cand = min(min(sm.imin[0], sm.imin[1]), min(sm.imin[2], sm.imin[3]));
             if (cand == 0x7fffffff) break;
-            const int kv = cand / N, ku = cand - kv * N;
-            long long acc = 0;
-            for (int i = tid; i < NN; i += kThreads) {
-                int y = i / N, x = i - y * N;
-                int yy = y + kv; if (yy >= N) yy -= N;
-                int xx = x + ku; if (xx >= N) xx -= N;
-                acc += (int)sm.wr[i] * (int)sm.wt[yy * N + xx];
-            }
-            acc = wave_sum(acc);
-            __syncthreads();
-            if (lane == 0) sm.red[wave][0] = acc;
-            __syncthreads();
-            acc = sm.red[0][0] + sm.red[1][0] + sm.red[2][0] + sm.red[3][0];
-            if (acc > bestI) { bestI = acc; bestk = cand; }
+            const int I = exact_corr_dft(sm, N, cand % N, cand / N);
+            if (I > bestI) { bestI = I; peak = cand; }
             last = cand;
         }
-        peak = bestk;
-        zn = (float)(((double)bestI - (double)sr * (double)st / (double)NN) / sqrt(nr * nt));
     }
     if (tid == 0) {
-        int iu = peak % N, iv = peak / N;                   // :316-322
-        if (iu > half) iu -= N;
-        if (iv > half) iv -= N;
+        int iu, iv;
+        wrap_peak(peak, N, iu, iv);
         p.u[poi] = (float)iu;
         p.v[poi] = (float)iv;
-        p.zncc[poi] = zn;
+        p.zncc[poi] = (float)(((double)bestI - (double)Sr * (double)St / (double)NN) / sqrt(nr * nt));
         if (p.status) p.status[poi] = TWDIC_FFTCC_OK;
     }
 }
 
-hipError_t launch_fftcc(const FftccParams& p, hipStream_t st)
+// variant: occupancy target of the register engine (waves per SIMD the
+// compiler budgets registers for): 2 (no spills), 3 or 4.
+hipError_t launch_fftcc(const FftccParams& p, int variant, hipStream_t st)
 {
     if (p.n <= 0) return hipSuccess;
-    if (p.N == 64)
-        hipLaunchKernelGGL(k_fftcc<16>, dim3(p.n), dim3(kThreads), 0, st, p);
-    else if (p.N == 32)
-        hipLaunchKernelGGL(k_fftcc<8>, dim3(p.n), dim3(kThreads), 0, st, p);
-    else
-        hipLaunchKernelGGL(k_fftcc<0>, dim3(p.n), dim3(kThreads), 0, st, p);
+    const int pairs = (p.n + 1) / 2;
+    if (p.N == 64) {
+        if (variant == 3)
+            hipLaunchKernelGGL((k_fftcc_fft<16, 3>), dim3(pairs), dim3(256), 0, st, p);
+        else if (variant == 4)
+            hipLaunchKernelGGL((k_fftcc_fft<16, 4>), dim3(pairs), dim3(256), 0, st, p);
+        else
+            hipLaunchKernelGGL((k_fftcc_fft<16, 2>), dim3(pairs), dim3(256), 0, st, p);
+    } else if (p.N == 32) {
+        hipLaunchKernelGGL((k_fftcc_fft<8, 4>), dim3(pairs), dim3(128), 0, st, p);
+    } else
+        hipLaunchKernelGGL(k_fftcc_dft, dim3(p.n), dim3(kDftThreads), 0, st, p);
     return hipGetLastError();
 }
 
-// Generic-engine launch for any even N (used to cross-check the fast path).
 hipError_t launch_fftcc_generic(const FftccParams& p, hipStream_t st)
 {
     if (p.n <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_fftcc<0>, dim3(p.n), dim3(kThreads), 0, st, p);
+    hipLaunchKernelGGL(k_fftcc_dft, dim3(p.n), dim3(kDftThreads), 0, st, p);
     return hipGetLastError();
 }
 

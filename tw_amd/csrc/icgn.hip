@@ -41,9 +41,9 @@ __device__ __forceinline__ float warp_coord(int poi, float w0, float w1, float w
 }
 
 // Bicubic value from the materialised LUT cell (16 floats, 4k+l ~ y^k x^l).
-__device__ __forceinline__ float interp_lut(const float* __restrict__ lut, size_t cell, float tx, float ty)
+__device__ __forceinline__ float interp_lut(const float* __restrict__ lut, uint32_t cell, float tx, float ty)
 {
-    const float4* c = reinterpret_cast<const float4*>(lut + cell * 16);
+    const float4* c = reinterpret_cast<const float4*>(lut) + (size_t)cell * 4;
     float4 r0 = c[0], r1 = c[1], r2 = c[2], r3 = c[3];
     float a0 = fmaf(fmaf(fmaf(r0.w, tx, r0.z), tx, r0.y), tx, r0.x);
     float a1 = fmaf(fmaf(fmaf(r1.w, tx, r1.z), tx, r1.y), tx, r1.x);
@@ -53,37 +53,47 @@ __device__ __forceinline__ float interp_lut(const float* __restrict__ lut, size_
 }
 
 // The same bicubic interpolant evaluated from the 4x4 pixel neighbourhood:
-// the LUT polynomial with central-difference corner derivatives is the
-// tensor-product Catmull-Rom spline.  Differences to the cell's top-left
-// pixel keep constant regions exact.
+// the LUT polynomial built from central-difference corner derivatives is the
+// tensor-product Catmull-Rom spline.  Weights in Horner form.
 __device__ __forceinline__ void catmull_rom_w(float t, float w[4])
 {
-    float t2 = t * t;
-    w[0] = 0.5f * (t * (2.f * t - t2 - 1.f));
-    w[1] = 0.5f * (t2 * (3.f * t - 5.f) + 2.f);
-    w[2] = 0.5f * (t * ((4.f - 3.f * t) * t + 1.f));
-    w[3] = 0.5f * (t2 * (t - 1.f));
+    const float t2 = t * t;
+    w[0] = t * fmaf(t, fmaf(-0.5f, t, 1.f), -0.5f);
+    w[1] = fmaf(t2, fmaf(1.5f, t, -2.5f), 1.f);
+    w[2] = t * fmaf(t, fmaf(-1.5f, t, 2.f), 0.5f);
+    w[3] = t2 * fmaf(0.5f, t, -0.5f);
 }
 
+__device__ __forceinline__ float row_dot(uint32_t w, const float wx[4])
+{
+    float s = wx[0] * (float)(w & 255u);
+    s = fmaf(wx[1], (float)((w >> 8) & 255u), s);
+    s = fmaf(wx[2], (float)((w >> 16) & 255u), s);
+    return fmaf(wx[3], (float)(w >> 24), s);
+}
+
+// One unaligned dword per neighbourhood row.  A uniform neighbourhood returns
+// its value exactly (as the LUT does: all higher coefficients vanish), so
+// uniform target regions keep the reference's DarkSubset behaviour.
 __device__ __forceinline__ float interp_fused(const uint8_t* __restrict__ tar, int stride, int ix, int iy,
                                               float tx, float ty)
 {
-    const uint8_t* b = tar + (size_t)(iy - 1) * stride + (ix - 1);
+    const uint8_t* b = tar + (uint32_t)((iy - 1) * stride + (ix - 1));
+    uint32_t r0, r1, r2, r3;
+    __builtin_memcpy(&r0, b, 4);
+    __builtin_memcpy(&r1, b + stride, 4);
+    __builtin_memcpy(&r2, b + 2 * stride, 4);
+    __builtin_memcpy(&r3, b + 3 * stride, 4);
     float wx[4], wy[4];
     catmull_rom_w(tx, wx);
     catmull_rom_w(ty, wy);
-    const int c = b[stride + 1];
-    float acc = 0.f;
-#pragma unroll
-    for (int r = 0; r < 4; ++r) {
-        const uint8_t* row = b + r * stride;
-        float s = wx[0] * (float)(row[0] - c);
-        s = fmaf(wx[1], (float)(row[1] - c), s);
-        s = fmaf(wx[2], (float)(row[2] - c), s);
-        s = fmaf(wx[3], (float)(row[3] - c), s);
-        acc = fmaf(wy[r], s, acc);
-    }
-    return (float)c + acc;
+    float acc = wy[0] * row_dot(r0, wx);
+    acc = fmaf(wy[1], row_dot(r1, wx), acc);
+    acc = fmaf(wy[2], row_dot(r2, wx), acc);
+    acc = fmaf(wy[3], row_dot(r3, wx), acc);
+    const uint32_t c = (r1 >> 8) & 255u, rep = c * 0x01010101u;
+    const bool uniform = (r0 == rep) & (r1 == rep) & (r2 == rep) & (r3 == rep);
+    return uniform ? (float)c : acc;
 }
 
 // W(p) o W(dp)^-1 and p from W: TW_paDIC_ICGN2D_CPU.cpp:713-745 / :875-907,
@@ -126,15 +136,80 @@ __device__ __forceinline__ double conv_norm(const float d[6], int Sx, int Sy)
 
 }  // namespace
 
+// Workgroups are dispatched round-robin over the 8 XCDs; remap so each XCD
+// walks one contiguous range of POIs and its L2 holds one band of the image.
+__device__ __forceinline__ int xcd_block(int b, int nb)
+{
+    const int x = b & 7, slot = b >> 3, q = nb >> 3, r = nb & 7;
+    return x * q + min(x, r) + slot;
+}
+
+// One pass over the subset at the current warp.  INT: first pass at integer
+// target pixels (:610-637); otherwise bicubic (:757-792).  Returns false when
+// a warped pixel leaves the ROI.  Accumulates sum T, sum T^2 and the six
+// sum SD_k T / 2 (the gradient plane stores 2 Rx, 2 Ry) in FP64.
+template <int INTERP, bool INT>
+__device__ __forceinline__ bool subset_pass(const IcgnParams& P, int px, int py, const float Wm[6], int lane,
+                                            double acc[8])
+{
+    const int Sx = P.Sx, Sy = P.Sy, Wsub = P.Wsub, s = P.s;
+    const int sx0 = P.sx0, sy0 = P.sy0, rw = P.rw, rh = P.rh;
+    const int q64 = 64 / Wsub, r64 = 64 - q64 * Wsub;
+    int l = lane / Wsub, m = lane - (lane / Wsub) * Wsub;
+    const uint32_t* g2 = P.g2 + (size_t)(py - Sy - sy0) * rw + (px - Sx - sx0);
+    double st = 0.0, stt = 0.0, a0 = 0.0, a1 = 0.0, a2 = 0.0, b0 = 0.0, b1 = 0.0, b2 = 0.0;
+    bool out = false;
+    for (int p = lane; p < s; p += 64) {
+        const float dx = (float)(m - Sx), dy = (float)(l - Sy);
+        const float wx = warp_coord(px, Wm[0], Wm[1], Wm[2], dx, dy);
+        const float wy = warp_coord(py, Wm[3], Wm[4], Wm[5], dx, dy);
+        const int ix = (int)wx, iy = (int)wy;          // truncation, as int()
+        const bool in = ix >= sx0 && iy >= sy0 && ix < sx0 + rw && iy < sy0 + rh;
+        out |= !in;
+        float T = 0.f;
+        if (in) {
+            if (INT) {
+                T = (float)P.tar[(uint32_t)(iy * P.stride + ix)];                  // :629
+            } else if (ix != sx0 + rw - 1 && iy != sy0 + rh - 1) {   // the CPU LUT's last row/column is zero
+                const float tx = wx - (float)ix, ty = wy - (float)iy;            // :771-772
+                if (INTERP == TWDIC_INTERP_LUT)
+                    T = interp_lut(P.lut, (uint32_t)((iy - sy0) * rw + (ix - sx0)), tx, ty);
+                else
+                    T = interp_fused(P.tar, P.stride, ix, iy, tx, ty);
+            }
+        }
+        const uint32_t g = g2[l * rw + m];
+        const double gx = (double)(int16_t)(g & 0xffffu), gy = (double)(int16_t)(g >> 16);
+        const double Td = T;
+        st += Td;
+        stt = fma(Td, Td, stt);
+        const double a = gx * Td, b = gy * Td;
+        a0 += a;
+        a1 = fma(a, (double)dx, a1);
+        a2 = fma(a, (double)dy, a2);
+        b0 += b;
+        b1 = fma(b, (double)dx, b1);
+        b2 = fma(b, (double)dy, b2);
+        m += r64; l += q64;
+        if (m >= Wsub) { m -= Wsub; l += 1; }
+    }
+    acc[0] = st; acc[1] = stt;
+    acc[2] = 0.5 * a0; acc[3] = 0.5 * a1; acc[4] = 0.5 * a2;
+    acc[5] = 0.5 * b0; acc[6] = 0.5 * b1; acc[7] = 0.5 * b2;
+    return !__any(out);
+}
+
 template <int INTERP>
 __global__ __launch_bounds__(256) void k_icgn(IcgnParams P)
 {
     const int lane = threadIdx.x & 63;
-    const int poi = blockIdx.x * 4 + (threadIdx.x >> 6);
+    // the POI index is wave-uniform: per-POI state goes to scalar registers
+    const int poi = __builtin_amdgcn_readfirstlane(xcd_block(blockIdx.x, gridDim.x) * 4 + (threadIdx.x >> 6));
     if (poi >= P.n) return;
-    const int py = P.poi[2 * poi], px = P.poi[2 * poi + 1];
-    const int Sx = P.Sx, Sy = P.Sy, Wsub = P.Wsub, s = P.s;
-    const int sx0 = P.sx0, sy0 = P.sy0, rw = P.rw, rh = P.rh;
+    const int py = __builtin_amdgcn_readfirstlane(P.poi[2 * poi]);
+    const int px = __builtin_amdgcn_readfirstlane(P.poi[2 * poi + 1]);
+    const int Sx = P.Sx, Sy = P.Sy;
+    const double dn = (double)P.s;
 
     float Pp[6] = {P.u[poi], 0.f, 0.f, P.v[poi], 0.f, 0.f};
     float Wm[6] = {1.f + Pp[1], Pp[2], Pp[0], Pp[4], 1.f + Pp[5], Pp[3]};  // :605-607
@@ -147,103 +222,55 @@ __global__ __launch_bounds__(256) void k_icgn(IcgnParams P)
     if (refflag == TWDIC_ICGN_DARK_SUBSET || refflag == TWDIC_ICGN_OUT_OF_ROI) {
         flag = refflag;
     } else {
-        double L[21], A[6], B[6];
         const double normR = rs[RS_NORM], meanR = rs[RS_MEAN];
-#pragma unroll
-        for (int i = 0; i < 21; ++i) L[i] = rs[RS_L + i];
-#pragma unroll
-        for (int i = 0; i < 6; ++i) {
-            A[i] = rs[RS_A + i];
-            B[i] = centred(rs[RS_B + i], meanR, A[i]);   // sum SD_k (R - mean R)
-        }
-        const int q64 = 64 / Wsub, r64 = 64 - q64 * Wsub;
-        const int l0 = lane / Wsub, m0 = lane - l0 * Wsub;
-
         for (int it = 0;; ++it) {
             if (it >= 1) iters = it + 1;                       // ++iNumIterations (:753)
-            double st = 0.0, stt = 0.0, sd0 = 0.0, sd1 = 0.0, sd2 = 0.0, sd3 = 0.0, sd4 = 0.0, sd5 = 0.0;
-            bool out = false;
-            int l = l0, m = m0;
-            for (int p = lane; p < s; p += 64) {
-                const float dx = (float)(m - Sx), dy = (float)(l - Sy);
-                const float wx = warp_coord(px, Wm[0], Wm[1], Wm[2], dx, dy);
-                const float wy = warp_coord(py, Wm[3], Wm[4], Wm[5], dx, dy);
-                const int ix = (int)wx, iy = (int)wy;          // truncation, as int()
-                const bool in = ix >= sx0 && iy >= sy0 && ix < sx0 + rw && iy < sy0 + rh;
-                out |= !in;
-                float T = 0.f;
-                if (in) {
-                    if (it == 0) {
-                        T = (float)P.tar[(size_t)iy * P.stride + ix];          // :629
-                    } else if (ix == sx0 + rw - 1 || iy == sy0 + rh - 1) {
-                        T = 0.f;   // the CPU LUT's last row/column is zero (:643-699)
-                    } else {
-                        const float tx = wx - (float)ix, ty = wy - (float)iy;   // :771-772
-                        if (INTERP == TWDIC_INTERP_LUT)
-                            T = interp_lut(P.lut, (size_t)(iy - sy0) * rw + (ix - sx0), tx, ty);
-                        else
-                            T = interp_fused(P.tar, P.stride, ix, iy, tx, ty);
-                    }
-                }
-                const size_t gi = (size_t)(py - Sy + l - sy0) * rw + (px - Sx + m - sx0);
-                const double gx = P.rx[gi], gy = P.ry[gi];
-                const double Td = T;
-                st += Td;
-                stt = fma(Td, Td, stt);
-                const double a = gx * Td, b = gy * Td;
-                sd0 += a;
-                sd1 = fma(a, (double)dx, sd1);
-                sd2 = fma(a, (double)dy, sd2);
-                sd3 += b;
-                sd4 = fma(b, (double)dx, sd4);
-                sd5 = fma(b, (double)dy, sd5);
-                m += r64; l += q64;
-                if (m >= Wsub) { m -= Wsub; l += 1; }
-            }
-            if (__any(out)) { flag = TWDIC_ICGN_OUT_OF_ROI; break; }   // :632-635, :787-790
-            st = wave_sum(st); stt = wave_sum(stt);
-            sd0 = wave_sum(sd0); sd1 = wave_sum(sd1); sd2 = wave_sum(sd2);
-            sd3 = wave_sum(sd3); sd4 = wave_sum(sd4); sd5 = wave_sum(sd5);
+            double acc[8];
+            const bool inside = it == 0 ? subset_pass<INTERP, true>(P, px, py, Wm, lane, acc)
+                                        : subset_pass<INTERP, false>(P, px, py, Wm, lane, acc);
+            if (!inside) { flag = TWDIC_ICGN_OUT_OF_ROI; break; }         // :632-635, :787-790
+#pragma unroll
+            for (int i = 0; i < 8; ++i) acc[i] = wave_sum(acc[i]);
             double meanT, normT;
-            mean_spread(st, stt, (double)s, meanT, normT);
+            mean_spread(acc[0], acc[1], dn, meanT, normT);
             if (fabsf((float)meanT) <= FLT_EPSILON) { flag = TWDIC_ICGN_DARK_SUBSET; break; }   // :638-640
             if ((float)normT <= FLT_EPSILON) { flag = TWDIC_ICGN_DARK_SUBSET; break; }          // :651-653
-            if (it == 0 && refflag == TWDIC_ICGN_SINGULAR_HESSIAN) {   // :688-693
+            if (it == 0 && refflag == TWDIC_ICGN_SINGULAR_HESSIAN) {                              // :688-693
                 flag = TWDIC_ICGN_SINGULAR_HESSIAN;
                 break;
             }
-            const double nor = normR / normT;
             // RHS_k = sum SD_k ((R_s/T_s) (T - mean T) - (R - mean R))   (:657-667)
-            double r[6] = {__fma_rn(nor, centred(sd0, meanT, A[0]), -B[0]),
-                           __fma_rn(nor, centred(sd1, meanT, A[1]), -B[1]),
-                           __fma_rn(nor, centred(sd2, meanT, A[2]), -B[2]),
-                           __fma_rn(nor, centred(sd3, meanT, A[3]), -B[3]),
-                           __fma_rn(nor, centred(sd4, meanT, A[4]), -B[4]),
-                           __fma_rn(nor, centred(sd5, meanT, A[5]), -B[5])};
-            // L y = r, L^T x = y (spotrs 'L', :683, :853) in FP64
+            const double nor = normR / normT;
+            double r[6];
+#pragma unroll
+            for (int k = 0; k < 6; ++k)
+                r[k] = __fma_rn(nor, centred(acc[2 + k], meanT, rs[RS_A + k]),
+                                -centred(rs[RS_B + k], meanR, rs[RS_A + k]));
+            // L y = r, L^T x = y with the FP64 factor (spotrs 'L', :683, :853)
 #pragma unroll
             for (int i = 0; i < 6; ++i) {
                 double v = r[i];
 #pragma unroll
-                for (int k = 0; k < i; ++k) v -= L[i * (i + 1) / 2 + k] * r[k];
-                r[i] = v / L[i * (i + 1) / 2 + i];
+                for (int k = 0; k < i; ++k) v -= rs[RS_L + i * (i + 1) / 2 + k] * r[k];
+                r[i] = v * rs[RS_RDIAG + i];
             }
 #pragma unroll
             for (int i = 5; i >= 0; --i) {
                 double v = r[i];
 #pragma unroll
-                for (int k = i + 1; k < 6; ++k) v -= L[k * (k + 1) / 2 + i] * r[k];
-                r[i] = v / L[i * (i + 1) / 2 + i];
+                for (int k = i + 1; k < 6; ++k) v -= rs[RS_L + k * (k + 1) / 2 + i] * r[k];
+                r[i] = v * rs[RS_RDIAG + i];
             }
             float d[6];
 #pragma unroll
             for (int i = 0; i < 6; ++i) d[i] = (float)r[i];
             const float den = warp_den(d);
-            if (fabsf(den) <= FLT_EPSILON) { flag = TWDIC_ICGN_SINGULAR_WARP; break; }
+            if (fabsf(den) <= FLT_EPSILON) { flag = TWDIC_ICGN_SINGULAR_WARP; break; }        // :875-877
             warp_update(Pp, d, den, Wm);
-            dpn = (float)conv_norm(d, Sx, Sy);
+            const double nrm = conv_norm(d, Sx, Sy);
+            dpn = (float)nrm;
             if (it == 0) iters = 1;                                    // :749
-            if (!(iters < P.max_iter && conv_norm(d, Sx, Sy) >= (double)P.delta_p)) break;
+            if (!(iters < P.max_iter && nrm >= (double)P.delta_p)) break;
         }
     }
     if (lane == 0) {

@@ -34,12 +34,14 @@ hipError_t launch_poi_grid(int32_t* poi, int start_x, int start_y, int nx, int n
 
 // ---------------------------------------------------------------------------
 // Gradients: GradientXY_2Images_s (TW_utils.cpp:260-273), one image.
-// Thread per pixel, x fastest; stores are fully coalesced.
+// Thread per pixel, x fastest; stores are fully coalesced.  g2 (optional)
+// receives the doubled gradients (2 Gx, 2 Gy) as a packed int16 pair, the
+// 4-byte-per-pixel form the IC-GN kernels read.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_gradients(const uint8_t* __restrict__ img, int stride,
                                                    int sx0, int sy0, int rw, int rh,
                                                    float* __restrict__ gx, float* __restrict__ gy,
-                                                   float* __restrict__ gxy)
+                                                   float* __restrict__ gxy, uint32_t* __restrict__ g2)
 {
     int x = blockIdx.x * 64 + (threadIdx.x & 63);
     int y = blockIdx.y * 4 + (threadIdx.x >> 6);
@@ -49,6 +51,7 @@ __global__ __launch_bounds__(256) void k_gradients(const uint8_t* __restrict__ i
     size_t o = (size_t)y * rw + x;
     gx[o] = 0.5f * (float)(r - l);
     gy[o] = 0.5f * (float)(d - u);
+    if (g2) g2[o] = (uint32_t)(uint16_t)(int16_t)(r - l) | ((uint32_t)(uint16_t)(int16_t)(d - u) << 16);
     if (gxy) {
         int ur = c[-stride + 1], ul = c[-stride - 1], dr = c[stride + 1], dl = c[stride - 1];
         gxy[o] = 0.25f * (float)(dr - ur - dl + ul);
@@ -56,10 +59,10 @@ __global__ __launch_bounds__(256) void k_gradients(const uint8_t* __restrict__ i
 }
 
 hipError_t launch_gradients(const uint8_t* img, int stride, int sx0, int sy0, int rw, int rh,
-                            float* gx, float* gy, float* gxy, hipStream_t st)
+                            float* gx, float* gy, float* gxy, uint32_t* g2, hipStream_t st)
 {
     dim3 grid((rw + 63) / 64, (rh + 3) / 4);
-    hipLaunchKernelGGL(k_gradients, grid, dim3(256), 0, st, img, stride, sx0, sy0, rw, rh, gx, gy, gxy);
+    hipLaunchKernelGGL(k_gradients, grid, dim3(256), 0, st, img, stride, sx0, sy0, rw, rh, gx, gy, gxy, g2);
     return hipGetLastError();
 }
 
@@ -179,8 +182,8 @@ __global__ __launch_bounds__(256) void k_ref_state(IcgnParams P)
     for (int p = lane; p < s; p += kWave) {
         int gy = py - P.Sy + l, gxp = px - P.Sx + m;
         int R = P.ref[(size_t)gy * P.stride + gxp];
-        size_t gi = (size_t)(gy - P.sy0) * P.rw + (gxp - P.sx0);
-        double rx = P.rx[gi], ry = P.ry[gi];
+        const uint32_t g = P.g2[(size_t)(gy - P.sy0) * P.rw + (gxp - P.sx0)];
+        double rx = 0.5 * (double)(int16_t)(g & 0xffffu), ry = 0.5 * (double)(int16_t)(g >> 16);
         double dx = (double)(m - P.Sx), dy = (double)(l - P.Sy);
         double sd[6] = {rx, rx * dx, rx * dy, ry, ry * dx, ry * dy};
         int t = 0;
@@ -238,6 +241,8 @@ __global__ __launch_bounds__(256) void k_ref_state(IcgnParams P)
         rs[RS_NORM] = spread;
         rs[RS_FLAG] = flag;
         rs[RS_MEAN] = mean;
+#pragma unroll
+        for (int i = 0; i < 6; ++i) rs[RS_RDIAG + i] = flag == 0.0 ? 1.0 / L[i * (i + 1) / 2 + i] : 0.0;
     }
 }
 

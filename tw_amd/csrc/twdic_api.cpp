@@ -4,6 +4,7 @@
 
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <new>
 
@@ -98,7 +99,7 @@ twdic_status ensure_ref_grads(twdic_ctx* c)
     if (c->ref_grads_valid) return TWDIC_OK;
     const twdic_config& g = c->cfg;
     TW_HIP(c, launch_gradients(c->d_ref, c->dstride, g.roi_x, g.roi_y, g.roi_w, g.roi_h, c->d_rx, c->d_ry,
-                               nullptr, c->stream));
+                               nullptr, c->d_g2, c->stream));
     c->ref_grads_valid = true;
     return TWDIC_OK;
 }
@@ -112,7 +113,7 @@ IcgnParams icgn_params(twdic_ctx* c, const int32_t* poi, int n, float* u, float*
     p.sx0 = g.roi_x; p.sy0 = g.roi_y; p.rw = g.roi_w; p.rh = g.roi_h;
     p.Sx = g.sx; p.Sy = g.sy; p.Wsub = 2 * g.sx + 1; p.Hsub = 2 * g.sy + 1; p.s = p.Wsub * p.Hsub;
     p.max_iter = g.max_iter; p.delta_p = g.delta_p;
-    p.rx = c->d_rx; p.ry = c->d_ry; p.lut = c->d_lut;
+    p.g2 = c->d_g2; p.lut = c->d_lut;
     p.refstate = c->d_refstate;
     p.poi = poi; p.n = n;
     p.u = u; p.v = v; p.p6 = p6; p.iters = iters; p.flags = flags; p.dpnorm = dpnorm;
@@ -145,6 +146,7 @@ twdic_status twdic_create(const twdic_config* cfg, int device, twdic_ctx** out)
     c->device = device;
     num_poi(*cfg, &c->nx, &c->ny);
     c->dstride = (cfg->img_w + 63) & ~63;
+    if (const char* v = std::getenv("TWDIC_FFTCC_VARIANT")) c->fftcc_variant = std::atoi(v);
     twdic_status st = TWDIC_OK;
     auto bail = [&](hipError_t e, const char* w) {
         hip_fail(c, e, w);
@@ -160,6 +162,7 @@ twdic_status twdic_create(const twdic_config* cfg, int device, twdic_ctx** out)
         {(void**)&c->d_rx, roi * 4}, {(void**)&c->d_ry, roi * 4},
         {(void**)&c->d_tx, roi * 4}, {(void**)&c->d_ty, roi * 4}, {(void**)&c->d_txy, roi * 4},
         {(void**)&c->d_lut, roi * 64}, {(void**)&c->d_fft_tw, sizeof(float2) * kMaxFftN},
+        {(void**)&c->d_g2, roi * 4},
     };
     for (auto& a : allocs) {
         if (st != TWDIC_OK) break;
@@ -193,7 +196,7 @@ twdic_status twdic_destroy(twdic_ctx* c)
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     void* bufs[] = {c->d_ref, c->d_tar, c->d_rx, c->d_ry, c->d_tx, c->d_ty, c->d_txy, c->d_lut,
-                    c->d_fft_tw, c->d_refstate};
+                    c->d_fft_tw, c->d_refstate, c->d_g2};
     for (void* b : bufs)
         if (b) (void)hipFree(b);
     if (c->h_stage) (void)hipHostFree(c->h_stage);
@@ -261,7 +264,7 @@ twdic_status twdic_fftcc(twdic_ctx* c, const int32_t* d_poi, int n, float* d_u, 
     p.poi = d_poi; p.n = n; p.N = c->cfg.fft_n; p.tw = c->d_fft_tw;
     p.tol = 1e-4f;
     p.u = d_u; p.v = d_v; p.zncc = d_zncc; p.status = d_status;
-    TW_HIP(c, launch_fftcc(p, c->stream));
+    TW_HIP(c, launch_fftcc(p, c->fftcc_variant, c->stream));
     return TWDIC_OK;
 }
 
@@ -273,7 +276,7 @@ twdic_status twdic_precompute_target(twdic_ctx* c)
     if (st != TWDIC_OK) return st;
     const twdic_config& g = c->cfg;
     TW_HIP(c, launch_gradients(c->d_tar, c->dstride, g.roi_x, g.roi_y, g.roi_w, g.roi_h, c->d_tx, c->d_ty,
-                               c->d_txy, c->stream));
+                               c->d_txy, nullptr, c->stream));
     TW_HIP(c, launch_bicubic_lut(c->d_tar, c->dstride, g.roi_x, g.roi_y, g.roi_w, g.roi_h, c->d_lut, c->stream));
     c->tar_pre_valid = true;
     return TWDIC_OK;

@@ -14,7 +14,7 @@ namespace twdic {
 
 constexpr int kWave = 64;             // CDNA wavefront width
 constexpr int kMaxFftN = 64;          // largest FFT-CC window edge held in LDS
-constexpr int kRefStateStride = 36;   // doubles of per-reference state per POI
+constexpr int kRefStateStride = 48;   // doubles of per-reference state per POI (384 B)
 
 // Per-reference, per-POI state (doubles, kRefStateStride per POI):
 //   [0..20]  FP64 Cholesky factor L of the 6x6 Hessian, packed lower, row-major
@@ -24,7 +24,8 @@ constexpr int kRefStateStride = 36;   // doubles of per-reference state per POI
 //   [34]     reference flag: 0 ok, TWDIC_ICGN_DARK_SUBSET, TWDIC_ICGN_SINGULAR_HESSIAN,
 //            TWDIC_ICGN_OUT_OF_ROI (subset leaves the ROI)
 //   [35]     mean R
-enum { RS_L = 0, RS_A = 21, RS_B = 27, RS_NORM = 33, RS_FLAG = 34, RS_MEAN = 35 };
+//   [36..41] 1 / L_ii (reciprocal pivots of the factor)
+enum { RS_L = 0, RS_A = 21, RS_B = 27, RS_NORM = 33, RS_FLAG = 34, RS_MEAN = 35, RS_RDIAG = 36 };
 
 // Mean and L2 spread of n samples from their exact-or-FP64 sums s1 = sum x,
 // s2 = sum x^2.  Shared by the reference-state and iteration kernels so that
@@ -66,8 +67,7 @@ struct IcgnParams {
     int Sx, Sy, Wsub, Hsub, s;
     int max_iter;
     float delta_p;
-    const float* rx;          // reference gradients, ROI-relative
-    const float* ry;
+    const uint32_t* g2;       // reference gradients (2Rx, 2Ry) as int16 pairs, ROI-relative
     const float* lut;         // target LUT [rh][rw][16]
     double* refstate;         // kRefStateStride doubles per POI
     const int32_t* poi;
@@ -82,12 +82,12 @@ struct IcgnParams {
 
 // Kernel launch wrappers (return hipError_t of the launch).
 hipError_t launch_gradients(const uint8_t* img, int stride, int sx0, int sy0, int rw, int rh,
-                            float* gx, float* gy, float* gxy, hipStream_t st);
+                            float* gx, float* gy, float* gxy, uint32_t* g2, hipStream_t st);
 hipError_t launch_bicubic_lut(const uint8_t* tar, int stride, int sx0, int sy0, int rw, int rh,
                               float* lut, hipStream_t st);
 hipError_t launch_poi_grid(int32_t* poi, int start_x, int start_y, int nx, int ny, int mx, int my,
                            int sx, int sy, int gx, int gy, hipStream_t st);
-hipError_t launch_fftcc(const FftccParams& p, hipStream_t st);
+hipError_t launch_fftcc(const FftccParams& p, int variant, hipStream_t st);
 hipError_t launch_ref_state(const IcgnParams& p, hipStream_t st);
 hipError_t launch_icgn(const IcgnParams& p, int interp, hipStream_t st);
 
@@ -107,6 +107,7 @@ struct twdic_ctx {
     float* d_ty = nullptr;
     float* d_txy = nullptr;
     float* d_lut = nullptr;
+    uint32_t* d_g2 = nullptr;         // packed (2Rx, 2Ry) int16 pairs
     float2* d_fft_tw = nullptr;
     bool ref_loaded = false, tar_loaded = false;
     bool ref_grads_valid = false;     // Rx/Ry match d_ref
@@ -118,6 +119,7 @@ struct twdic_ctx {
     uint64_t refstate_version = ~0ull;
     const int32_t* refstate_poi = nullptr;
     int refstate_n = -1;
+    int fftcc_variant = 3;            // tuned launch variant of the FFT-CC engine
     uint8_t* h_stage = nullptr;       // pinned staging for host images
     size_t h_stage_bytes = 0;
     hipEvent_t ev[16] = {};
