@@ -11,7 +11,8 @@
  * Pointer convention: arguments named d_* are device pointers (allocate them
  * with twdic_malloc or any HIP allocator on the context's device); arguments
  * named h_* are host pointers.  Images are 8-bit single channel, row-major,
- * with a row stride in bytes.
+ * with a row stride in bytes; the context's own device copies use a row
+ * stride equal to the image width (as the reference's device images do).
  *
  * The entries below replace the reference's class methods; each comment
  * names the reference interface it replaces (paths relative to the reference
@@ -143,10 +144,17 @@ twdic_status twdic_fftcc(twdic_ctx* ctx, const int32_t* d_poi_yx, int n,
  * d_iters (may be NULL): 1-based iteration count; untouched when a flag is
  * raised before the first update (as in the reference).
  * d_flags (may be NULL): ICGN2DFlag per POI.
- * d_dpnorm (may be NULL): weighted |dp| of the last update (audit aid). */
+ * d_dpnorm (may be NULL): weighted |dp| of the last update (audit aid).
+ * The per-POI reference state is cached under (reference upload, d_poi_yx,
+ * n); after rewriting a POI array in place call twdic_reset_poi_state. */
 twdic_status twdic_icgn(twdic_ctx* ctx, const int32_t* d_poi_yx, int n,
                         float* d_u, float* d_v, float* d_p6, int32_t* d_iters,
                         uint8_t* d_flags, float* d_dpnorm);
+
+/* Marks the cached per-POI reference state stale (rebuilt by the next
+ * twdic_icgn); the reference rebuilds it only on a reference update
+ * (TW_paDIC_cuICGN2D.cu:883-903), so it has no counterpart call. */
+twdic_status twdic_reset_poi_state(twdic_ctx* ctx);
 
 /* The plain per-frame step of FFTCCTWorkerThread::processFrameFFTCC_ICGN
  * (TW/TW_Core/fftcctworkerthread.cpp:445-461): FFT-CC then IC-GN on the

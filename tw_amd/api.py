@@ -61,6 +61,7 @@ _SIGS = {
     "twdic_fftcc": ([_VP, _VP, _I, _VP, _VP, _VP, _VP], _I),
     "twdic_icgn": ([_VP, _VP, _I, _VP, _VP, _VP, _VP, _VP, _VP], _I),
     "twdic_frame": ([_VP, _VP, _I, _VP, _VP, _VP, _VP, _VP, _VP], _I),
+    "twdic_reset_poi_state": ([_VP], _I),
     "twdic_get_buffer": ([_VP, _I, C.POINTER(_VP), C.POINTER(_SZ)], _I),
     "twdic_precompute_target": ([_VP], _I),
     "twdic_malloc": ([_VP, C.POINTER(_VP), _SZ], _I),

@@ -73,8 +73,12 @@ def build(verbose: bool = False) -> Path:
     tmp = LIB.with_suffix(".so.tmp")
     _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", str(tmp), *objs, "-lgomp"])
     os.replace(tmp, LIB)
+    # the TW_Core-style frame driver, linked against the library in-tree
+    exe = OUT_DIR / "tw_frame_loop"
+    _run([HIPCC, "-O2", "-std=c++17", f"-I{INCLUDE}", str(CPPSRC / "examples" / "frame_loop.cpp"),
+          "-o", str(exe), f"-L{OUT_DIR}", "-ltwdic", "-Wl,-rpath,$ORIGIN"])
     if verbose:
-        print(f"built {LIB}")
+        print(f"built {LIB} and {exe}")
     return LIB
 
 

@@ -48,10 +48,8 @@ const char* validate(const twdic_config& c)
 {
     if (c.img_w <= 0 || c.img_h <= 0) return "image size must be positive";
     if (c.roi_w < 2 || c.roi_h < 2) return "ROI must be at least 2x2";
-    // the gradients need a one-pixel frame around the ROI (TW_utils.cpp:246-257,
-    // with the reference's off-by-one corrected)
-    if (c.roi_x < 1 || c.roi_y < 1 || c.roi_x + c.roi_w + 1 > c.img_w || c.roi_y + c.roi_h + 1 > c.img_h)
-        return "Error: Not enough boundary pixels for gradients!";
+    if (c.roi_x < 0 || c.roi_y < 0 || c.roi_x + c.roi_w > c.img_w || c.roi_y + c.roi_h > c.img_h)
+        return "ROI must lie inside the image";
     if (c.sx < 1 || c.sy < 1 || c.sx > 60 || c.sy > 60) return "subset half size must be in [1, 60]";
     if (c.gx < 1 || c.gy < 1) return "grid spacing must be >= 1";
     if (c.mx < 0 || c.my < 0) return "margins must be >= 0";
@@ -93,9 +91,17 @@ twdic_status upload(twdic_ctx* c, uint8_t* dst, const uint8_t* img, int stride, 
     return TWDIC_OK;
 }
 
+// The gradients need a one-pixel frame around the ROI (TW_utils.cpp:246-257,
+// with the reference's off-by-one corrected); the reference throws there.
+bool gradient_frame_ok(const twdic_config& g)
+{
+    return g.roi_x >= 1 && g.roi_y >= 1 && g.roi_x + g.roi_w + 1 <= g.img_w && g.roi_y + g.roi_h + 1 <= g.img_h;
+}
+
 twdic_status ensure_ref_grads(twdic_ctx* c)
 {
     if (!c->ref_loaded) return fail(c, TWDIC_EINVAL, "reference image not set");
+    if (!gradient_frame_ok(c->cfg)) return fail(c, TWDIC_EINVAL, "Error: Not enough boundary pixels for gradients!");
     if (c->ref_grads_valid) return TWDIC_OK;
     const twdic_config& g = c->cfg;
     TW_HIP(c, launch_gradients(c->d_ref, c->dstride, g.roi_x, g.roi_y, g.roi_w, g.roi_h, c->d_rx, c->d_ry,
@@ -145,7 +151,7 @@ twdic_status twdic_create(const twdic_config* cfg, int device, twdic_ctx** out)
     c->cfg = *cfg;
     c->device = device;
     num_poi(*cfg, &c->nx, &c->ny);
-    c->dstride = (cfg->img_w + 63) & ~63;
+    c->dstride = cfg->img_w;
     if (const char* v = std::getenv("TWDIC_FFTCC_VARIANT")) c->fftcc_variant = std::atoi(v);
     twdic_status st = TWDIC_OK;
     auto bail = [&](hipError_t e, const char* w) {
@@ -316,6 +322,13 @@ twdic_status twdic_icgn(twdic_ctx* c, const int32_t* d_poi, int n, float* d_u, f
         c->refstate_n = n;
     }
     TW_HIP(c, launch_icgn(p, g.interp, c->stream));
+    return TWDIC_OK;
+}
+
+twdic_status twdic_reset_poi_state(twdic_ctx* c)
+{
+    if (!c) return TWDIC_EINVAL;
+    c->refstate_version = ~0ull;
     return TWDIC_OK;
 }
 
