@@ -5,7 +5,8 @@ Bars:
   * FFT-CC integer peak: bit-exact against the exact integer argmax; equal to
     the FP32 oracle except where the oracle's own FP32 surface has a near tie
     (top-2 gap < 1e-4 in ZNCC units), which is audited, not ignored;
-  * FFT-CC ZNCC: |GPU - FP32 oracle| <= 1e-5;
+  * FFT-CC ZNCC: within 1e-6 of the exact value and 3e-5 of the FP32 oracle
+    (whose own FFT rounding moves it by up to 2.9e-5 from exact at N = 64);
   * IC-GN (same initial guesses): flags equal; u, v within 1e-4 px of the
     FP32 oracle and 5e-5 px of the FP64 oracle; displacement gradients within
     2e-6 of the FP64 oracle and 3e-6 of the FP32 oracle, wherever both ran the
@@ -102,8 +103,10 @@ def test_fftcc_peaks_exact(c512, N):
     # against the FP32 restatement: equal except audited near ties
     diff = ok & ((g["u"] != f32["u"]) | (g["v"] != f32["v"]))
     assert np.all(f32["gap"][diff] < 1e-4), f32["gap"][diff]
-    assert np.max(np.abs(g["zncc"][ok] - f32["zncc"][ok])) <= 1e-5
-    assert np.max(np.abs(g["zncc"][ok] - ex["zncc"][ok])) <= 1e-5
+    # ZNCC is evaluated exactly (FP64 from integer sums): 1e-6 of the exact
+    # value; the FP32 CPU path's own FFT rounding moves its ZNCC up to 2.9e-5
+    assert np.max(np.abs(g["zncc"][ok] - ex["zncc"][ok])) <= 1e-6
+    assert np.max(np.abs(g["zncc"][ok] - f32["zncc"][ok])) <= 3e-5
 
 
 def test_fftcc_shift_recovery():
@@ -120,7 +123,7 @@ def test_fftcc_shift_recovery():
         same = _run_fftcc(ctx, poi)
     assert np.all(g["u"] == -5) and np.all(g["v"] == 3)
     ex = oracle.fftcc_exact(img, tar, poi, 32)
-    assert np.max(np.abs(g["zncc"] - ex["zncc"])) <= 1e-5
+    assert np.max(np.abs(g["zncc"] - ex["zncc"])) <= 1e-6
     # identical windows: zero shift, ZNCC 1
     assert np.all(same["u"] == 0) and np.all(same["v"] == 0)
     assert np.allclose(same["zncc"], 1.0, atol=1e-6)

@@ -336,3 +336,18 @@ def test_icgn_flags_by_construction():
     # subset leaving the ROI
     r = oracle.icgn(ref, ref, (1, 1, 254, 254), 10, 10, np.array([[8, 128]], np.int32), [0], [0])
     assert r["flags"][0] == oracle.OUT_OF_ROI
+
+
+def test_fp32_path_zncc_rounding():
+    """The FP32 CPU path's FFT-CC ZNCC against the exact value: the noise floor
+    of any ZNCC bar set against it (GPU bars: 1e-6 vs exact, 3e-5 vs FP32)."""
+    c = cases.speckle_512()
+    poi = c.poi()
+    p = poi[(poi[:, 0] >= 40) & (poi[:, 0] <= 470) & (poi[:, 1] >= 40) & (poi[:, 1] <= 470)][::3]
+    worst = 0.0
+    for N in (32, 64):
+        for tar in (c.tar, c.ref):
+            ex = oracle.fftcc_exact(c.ref, tar, p, N)
+            f = oracle.fftcc(c.ref, tar, p, N)
+            worst = max(worst, float(np.max(np.abs(f["zncc"] - ex["zncc"]))))
+    assert 5e-6 < worst < 3e-5

@@ -45,6 +45,23 @@ class Speckle(C.Structure):
                 ("boxes", C.c_void_p), ("nboxes", C.c_int)]
 
 
+class Pair(C.Structure):
+    _fields_ = [("d_ref", C.c_void_p), ("d_tar", C.c_void_p), ("stride", C.c_int), ("img_w", C.c_int),
+                ("img_h", C.c_int), ("roi_x", C.c_int), ("roi_y", C.c_int), ("roi_w", C.c_int), ("roi_h", C.c_int),
+                ("sx", C.c_int), ("sy", C.c_int), ("fft_n", C.c_int), ("max_iter", C.c_int), ("delta_p", C.c_float),
+                ("d_poi_yx", C.c_void_p), ("n", C.c_int), ("d_u", C.c_void_p), ("d_v", C.c_void_p),
+                ("d_zncc", C.c_void_p), ("d_p6", C.c_void_p), ("d_iters", C.c_void_p), ("d_flags", C.c_void_p)]
+
+
+class Tuning(C.Structure):
+    _fields_ = [("icgn_chunk", C.c_int), ("fftcc_variant", C.c_int), ("use_graph", C.c_int)]
+
+
+class BatchInfo(C.Structure):
+    _fields_ = [("npairs", C.c_int), ("total_pois", C.c_int64), ("fftcc_items", C.c_int64 * 3),
+                ("device_bytes", C.c_size_t), ("tuning", Tuning)]
+
+
 _lib = None
 
 _VP, _I, _SZ = C.c_void_p, C.c_int, C.c_size_t
@@ -70,6 +87,23 @@ _SIGS = {
     "twdic_memset": ([_VP, _VP, _I, _SZ], _I),
     "twdic_event_record": ([_VP, _I], _I),
     "twdic_event_elapsed": ([_VP, _I, _I, C.POINTER(C.c_float)], _I),
+    "twdic_batch_create": ([_I, C.POINTER(_VP)], _I),
+    "twdic_batch_destroy": ([_VP], _I),
+    "twdic_batch_last_error": ([_VP], C.c_char_p),
+    "twdic_batch_stream": ([_VP], _VP),
+    "twdic_batch_set_pairs": ([_VP, C.POINTER(Pair), _I], _I),
+    "twdic_batch_set_targets": ([_VP, C.POINTER(_VP)], _I),
+    "twdic_batch_set_references": ([_VP, C.POINTER(_VP)], _I),
+    "twdic_batch_run": ([_VP], _I),
+    "twdic_batch_run_icgn": ([_VP], _I),
+    "twdic_batch_sync": ([_VP], _I),
+    "twdic_batch_get_tuning": ([_VP, C.POINTER(Tuning)], _I),
+    "twdic_batch_set_tuning": ([_VP, C.POINTER(Tuning)], _I),
+    "twdic_batch_autotune": ([_VP, C.c_char_p, _I], _I),
+    "twdic_batch_load_tuning": ([_VP, C.c_char_p, C.POINTER(_I)], _I),
+    "twdic_batch_info_get": ([_VP, C.POINTER(BatchInfo)], _I),
+    "twdic_batch_record": ([_VP, _I], _I),
+    "twdic_batch_elapsed": ([_VP, _I, _I, C.POINTER(C.c_float)], _I),
     "twdic_synth_pair": ([C.POINTER(Speckle), C.POINTER(Deform), _VP, _VP, _I], _I),
     "twdic_synth_displacement": ([C.POINTER(Deform), C.c_double, C.c_double, C.POINTER(C.c_double)], None),
 }
@@ -285,3 +319,85 @@ def num_poi(cfg: Config) -> tuple[int, int]:
     nx, ny = C.c_int32(), C.c_int32()
     lib().twdic_num_poi(C.byref(cfg), C.byref(nx), C.byref(ny))
     return nx.value, ny.value
+
+
+class Batch:
+    """A twdic_batch: persistent-kernel processing of many image pairs.  Device
+    buffers are allocated through a helper Context-free allocator owned by the
+    batch object (the caller's arrays are borrowed)."""
+
+    def __init__(self, device: int = 0):
+        h = C.c_void_p()
+        st = lib().twdic_batch_create(device, C.byref(h))
+        if st != OK:
+            raise TwdicError(f"twdic_batch_create failed with status {st}")
+        self.h = h.value
+        self._keep = []
+
+    def _check(self, st, what=""):
+        if st != OK:
+            raise TwdicError(f"{what}: status {st}: {lib().twdic_batch_last_error(self.h).decode()}")
+
+    def close(self):
+        if self.h:
+            lib().twdic_batch_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def set_pairs(self, pairs: list):
+        arr = (Pair * len(pairs))(*pairs)
+        self._keep = [arr]
+        self._check(lib().twdic_batch_set_pairs(self.h, arr, len(pairs)), "set_pairs")
+
+    def set_targets(self, ptrs: list):
+        arr = (C.c_void_p * len(ptrs))(*ptrs)
+        self._check(lib().twdic_batch_set_targets(self.h, arr), "set_targets")
+
+    def set_references(self, ptrs: list):
+        arr = (C.c_void_p * len(ptrs))(*ptrs)
+        self._check(lib().twdic_batch_set_references(self.h, arr), "set_references")
+
+    def run(self):
+        self._check(lib().twdic_batch_run(self.h), "run")
+
+    def run_icgn(self):
+        self._check(lib().twdic_batch_run_icgn(self.h), "run_icgn")
+
+    def sync(self):
+        self._check(lib().twdic_batch_sync(self.h), "sync")
+
+    @property
+    def tuning(self) -> Tuning:
+        t = Tuning()
+        self._check(lib().twdic_batch_get_tuning(self.h, C.byref(t)), "get_tuning")
+        return t
+
+    def set_tuning(self, icgn_chunk=4, fftcc_variant=3, use_graph=1):
+        t = Tuning(icgn_chunk, fftcc_variant, use_graph)
+        self._check(lib().twdic_batch_set_tuning(self.h, C.byref(t)), "set_tuning")
+
+    def autotune(self, table_path: str | None = None, reps: int = 3):
+        self._check(lib().twdic_batch_autotune(self.h, table_path.encode() if table_path else None, reps), "autotune")
+
+    def load_tuning(self, table_path: str) -> bool:
+        found = C.c_int(0)
+        self._check(lib().twdic_batch_load_tuning(self.h, table_path.encode(), C.byref(found)), "load_tuning")
+        return bool(found.value)
+
+    def info(self) -> BatchInfo:
+        i = BatchInfo()
+        self._check(lib().twdic_batch_info_get(self.h, C.byref(i)), "info")
+        return i
+
+    def record(self, slot: int):
+        self._check(lib().twdic_batch_record(self.h, slot), "record")
+
+    def elapsed_ms(self, a: int, b: int) -> float:
+        ms = C.c_float()
+        self._check(lib().twdic_batch_elapsed(self.h, a, b, C.byref(ms)), "elapsed")
+        return ms.value

@@ -1,0 +1,470 @@
+// batch_api.cpp -- the batched form (include/twdic_batch.h): descriptors and
+// work lists for many image pairs, persistent-kernel stages, hipGraph replay
+// of the per-frame sequence, and the launch-parameter autotuner.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <map>
+#include <new>
+#include <sstream>
+#include <string>
+#include <vector>
+
+#include "twdic_batch.h"
+#include "twdic_internal.h"
+
+using namespace twdic;
+
+namespace {
+enum { CNT_STATE = 0, CNT_F64 = 1, CNT_F32 = 2, CNT_FDFT = 3, CNT_ICGN = 4, NCNT = 5 };
+}
+
+struct twdic_batch {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    std::vector<twdic_pair> pairs;
+    std::vector<PairDesc> h_desc;
+    PairDesc* d_desc = nullptr;
+    int2* d_items = nullptr;          // every POI as (pair, poi), pair-major: IC-GN and reference state
+    int2* d_fitems[3] = {};           // FFT-CC items per window class (N = 64, 32, other)
+    int64_t nf[3] = {};
+    int64_t total = 0;
+    uint32_t* d_g2 = nullptr;         // reference gradient planes of all pairs
+    double* d_state = nullptr;        // per-POI reference state of all pairs
+    float* d_zncc_scratch = nullptr;  // for pairs without a ZNCC output
+    std::map<int, float2*> tw;        // FFT twiddle tables per window edge
+    int* d_counters = nullptr;
+    bool state_valid = false;
+    twdic_batch_tuning tune{4, 3, 1};
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
+    bool graph_dirty = true;
+    hipEvent_t ev[16] = {};
+    size_t bytes = 0;
+};
+
+namespace {
+
+twdic_status bfail(twdic_batch* b, twdic_status st, const std::string& what)
+{
+    if (b) b->err = what;
+    return st;
+}
+
+#define TB_HIP(b, expr)                                                                       \
+    do {                                                                                      \
+        hipError_t e_ = (expr);                                                               \
+        if (e_ != hipSuccess) return bfail((b), TWDIC_EHIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+int fclass(int N) { return N == 64 ? 0 : N == 32 ? 1 : 2; }
+
+void release(twdic_batch* b)
+{
+    if (b->stream) (void)hipStreamSynchronize(b->stream);
+    if (b->graph_exec) (void)hipGraphExecDestroy(b->graph_exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    b->graph_exec = nullptr;
+    b->graph = nullptr;
+    void* bufs[] = {b->d_desc, b->d_items, b->d_fitems[0], b->d_fitems[1], b->d_fitems[2], b->d_g2, b->d_state,
+                    b->d_zncc_scratch, b->d_counters};
+    for (void* p : bufs)
+        if (p) (void)hipFree(p);
+    for (auto& kv : b->tw) (void)hipFree(kv.second);
+    b->tw.clear();
+    b->d_desc = nullptr; b->d_items = nullptr;
+    b->d_fitems[0] = b->d_fitems[1] = b->d_fitems[2] = nullptr;
+    b->d_g2 = nullptr; b->d_state = nullptr; b->d_zncc_scratch = nullptr; b->d_counters = nullptr;
+    b->bytes = 0;
+}
+
+const char* check_pair(const twdic_pair& p)
+{
+    if (!p.d_ref || !p.d_tar || (p.n > 0 && (!p.d_poi_yx || !p.d_u || !p.d_v))) return "null pointer in pair";
+    if (p.img_w <= 0 || p.img_h <= 0 || p.stride < p.img_w) return "bad image size/stride";
+    if (p.roi_x < 1 || p.roi_y < 1 || p.roi_w < 2 || p.roi_h < 2 || p.roi_x + p.roi_w + 1 > p.img_w ||
+        p.roi_y + p.roi_h + 1 > p.img_h)
+        return "ROI needs a one-pixel frame inside the image";
+    if (p.sx < 1 || p.sy < 1 || p.sx > 60 || p.sy > 60) return "subset half size must be in [1, 60]";
+    if (p.fft_n < 2 || p.fft_n > kMaxFftN || (p.fft_n & 1)) return "fft_n must be even and in [2, 64]";
+    if (p.max_iter < 1 || !(p.delta_p >= 0.f) || p.n < 0) return "bad iteration parameters or POI count";
+    return nullptr;
+}
+
+twdic_status upload_desc(twdic_batch* b)
+{
+    TB_HIP(b, hipMemcpyAsync(b->d_desc, b->h_desc.data(), sizeof(PairDesc) * b->h_desc.size(), hipMemcpyHostToDevice,
+                             b->stream));
+    TB_HIP(b, hipStreamSynchronize(b->stream));   // host descriptors may change right after
+    return TWDIC_OK;
+}
+
+twdic_status build_state(twdic_batch* b)
+{
+    for (size_t i = 0; i < b->pairs.size(); ++i) {
+        const IcgnParams& q = b->h_desc[i].icgn;
+        TB_HIP(b, launch_gradients(q.ref, q.stride, q.sx0, q.sy0, q.rw, q.rh, nullptr, nullptr, nullptr,
+                                   const_cast<uint32_t*>(q.g2), b->stream));
+    }
+    TB_HIP(b, launch_fill_zero(b->d_counters + CNT_STATE, 1, b->stream));
+    TB_HIP(b, launch_batch_ref_state(b->d_desc, b->d_items, (int)b->total, b->d_counters + CNT_STATE,
+                                     persistent_blocks_icgn(b->device), b->stream));
+    b->state_valid = true;
+    return TWDIC_OK;
+}
+
+// The per-frame sequence: reset counters, FFT-CC per window class, IC-GN.
+twdic_status enqueue_frame(twdic_batch* b, bool fftcc, bool icgn)
+{
+    TB_HIP(b, launch_fill_zero(b->d_counters + CNT_F64, 4, b->stream));
+    if (fftcc) {
+        const int Ns[3] = {64, 32, 0};
+        for (int c = 0; c < 3; ++c) {
+            if (!b->nf[c]) continue;
+            const int N = c == 2 ? 0 : Ns[c];
+            TB_HIP(b, launch_batch_fftcc(b->d_desc, b->d_fitems[c], (int)b->nf[c], b->d_counters + CNT_F64 + c, N,
+                                         b->tune.fftcc_variant,
+                                         persistent_blocks_fftcc(b->device, N, b->tune.fftcc_variant), b->stream));
+        }
+    }
+    if (icgn)
+        TB_HIP(b, launch_batch_icgn(b->d_desc, b->d_items, (int)b->total, b->d_counters + CNT_ICGN,
+                                    std::max(1, b->tune.icgn_chunk), persistent_blocks_icgn(b->device), b->stream));
+    return TWDIC_OK;
+}
+
+std::string signature(twdic_batch* b)
+{
+    hipDeviceProp_t prop;
+    (void)hipGetDeviceProperties(&prop, b->device);
+    std::map<std::pair<int, int>, int64_t> hist;   // (N, subset half size) -> POIs
+    for (const auto& p : b->pairs) hist[{p.fft_n, p.sx}] += p.n;
+    std::ostringstream os;
+    os << "arch=" << prop.gcnArchName << ";cus=" << prop.multiProcessorCount << ";pairs=" << b->pairs.size();
+    for (const auto& kv : hist) os << ";N" << kv.first.first << "S" << kv.first.second << "=" << kv.second;
+    return os.str();
+}
+
+}  // namespace
+
+extern "C" {
+
+twdic_status twdic_batch_create(int device, twdic_batch** out)
+{
+    if (!out) return TWDIC_EINVAL;
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0 || device < 0 || device >= ndev) return TWDIC_ENODEV;
+    twdic_batch* b = new (std::nothrow) twdic_batch();
+    if (!b) return TWDIC_ENOMEM;
+    b->device = device;
+    if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete b;
+        return TWDIC_EHIP;
+    }
+    for (auto& e : b->ev)
+        if (hipEventCreate(&e) != hipSuccess) {
+            twdic_batch_destroy(b);
+            return TWDIC_EHIP;
+        }
+    *out = b;
+    return TWDIC_OK;
+}
+
+twdic_status twdic_batch_destroy(twdic_batch* b)
+{
+    if (!b) return TWDIC_EINVAL;
+    (void)hipSetDevice(b->device);
+    release(b);
+    for (auto& e : b->ev)
+        if (e) (void)hipEventDestroy(e);
+    if (b->stream) (void)hipStreamDestroy(b->stream);
+    delete b;
+    return TWDIC_OK;
+}
+
+const char* twdic_batch_last_error(const twdic_batch* b) { return b ? b->err.c_str() : "null batch"; }
+void* twdic_batch_stream(twdic_batch* b) { return b ? (void*)b->stream : nullptr; }
+
+twdic_status twdic_batch_set_pairs(twdic_batch* b, const twdic_pair* pairs, int npairs)
+{
+    if (!b || npairs < 0 || (npairs > 0 && !pairs)) return TWDIC_EINVAL;
+    for (int i = 0; i < npairs; ++i)
+        if (const char* why = check_pair(pairs[i]))
+            return bfail(b, TWDIC_EINVAL, "pair " + std::to_string(i) + ": " + why);
+    (void)hipSetDevice(b->device);
+    release(b);
+    b->pairs.assign(pairs, pairs + npairs);
+    b->h_desc.assign(npairs, PairDesc{});
+    size_t g2_elems = 0, state_elems = 0, zn = 0;
+    int64_t total = 0;
+    for (const auto& p : b->pairs) {
+        g2_elems += (size_t)p.roi_w * p.roi_h;
+        state_elems += (size_t)p.n * kRefStateStride;
+        if (!p.d_zncc) zn += p.n;
+        total += p.n;
+    }
+    b->total = total;
+    auto alloc = [&](void** ptr, size_t bytes) -> hipError_t {
+        hipError_t e = hipMalloc(ptr, bytes ? bytes : 16);
+        if (e == hipSuccess) b->bytes += bytes;
+        return e;
+    };
+    TB_HIP(b, alloc((void**)&b->d_g2, g2_elems * sizeof(uint32_t)));
+    TB_HIP(b, alloc((void**)&b->d_state, state_elems * sizeof(double)));
+    TB_HIP(b, alloc((void**)&b->d_zncc_scratch, zn * sizeof(float)));
+    TB_HIP(b, alloc((void**)&b->d_desc, sizeof(PairDesc) * std::max(1, npairs)));
+    TB_HIP(b, alloc((void**)&b->d_items, sizeof(int2) * std::max<int64_t>(1, total)));
+    TB_HIP(b, alloc((void**)&b->d_counters, sizeof(int) * NCNT));
+    std::vector<int2> items;
+    std::vector<int2> fitems[3];
+    items.reserve(total);
+    size_t g2_off = 0, st_off = 0, zn_off = 0;
+    for (int i = 0; i < npairs; ++i) {
+        const twdic_pair& p = b->pairs[i];
+        if (!b->tw.count(p.fft_n)) {
+            float2 t[kMaxFftN];
+            for (int k = 0; k < p.fft_n; ++k) {
+                double a = -2.0 * M_PI * (double)k / (double)p.fft_n;
+                t[k] = make_float2((float)std::cos(a), (float)std::sin(a));
+            }
+            float2* d = nullptr;
+            TB_HIP(b, alloc((void**)&d, sizeof(float2) * kMaxFftN));
+            b->tw[p.fft_n] = d;
+            TB_HIP(b, hipMemcpy(d, t, sizeof(float2) * p.fft_n, hipMemcpyHostToDevice));
+        }
+        IcgnParams& q = b->h_desc[i].icgn;
+        q.ref = p.d_ref; q.tar = p.d_tar; q.stride = p.stride;
+        q.sx0 = p.roi_x; q.sy0 = p.roi_y; q.rw = p.roi_w; q.rh = p.roi_h;
+        q.Sx = p.sx; q.Sy = p.sy; q.Wsub = 2 * p.sx + 1; q.Hsub = 2 * p.sy + 1; q.s = q.Wsub * q.Hsub;
+        q.max_iter = p.max_iter; q.delta_p = p.delta_p;
+        q.g2 = b->d_g2 + g2_off; q.lut = nullptr;
+        q.refstate = b->d_state + st_off;
+        q.poi = p.d_poi_yx; q.n = p.n;
+        q.u = p.d_u; q.v = p.d_v; q.p6 = p.d_p6; q.iters = p.d_iters; q.flags = p.d_flags; q.dpnorm = nullptr;
+        FftccParams& f = b->h_desc[i].fftcc;
+        f.ref = p.d_ref; f.tar = p.d_tar; f.stride = p.stride; f.img_w = p.img_w; f.img_h = p.img_h;
+        f.poi = p.d_poi_yx; f.n = p.n; f.N = p.fft_n; f.tw = b->tw[p.fft_n]; f.tol = 1e-4f;
+        f.u = p.d_u; f.v = p.d_v; f.zncc = p.d_zncc ? p.d_zncc : b->d_zncc_scratch + zn_off; f.status = nullptr;
+        g2_off += (size_t)p.roi_w * p.roi_h;
+        st_off += (size_t)p.n * kRefStateStride;
+        if (!p.d_zncc) zn_off += p.n;
+        for (int k = 0; k < p.n; ++k) {
+            items.push_back(make_int2(i, k));
+            fitems[fclass(p.fft_n)].push_back(make_int2(i, k));
+        }
+    }
+    if (total) TB_HIP(b, hipMemcpy(b->d_items, items.data(), sizeof(int2) * total, hipMemcpyHostToDevice));
+    for (int c = 0; c < 3; ++c) {
+        b->nf[c] = (int64_t)fitems[c].size();
+        if (!b->nf[c]) continue;
+        TB_HIP(b, alloc((void**)&b->d_fitems[c], sizeof(int2) * fitems[c].size()));
+        TB_HIP(b, hipMemcpy(b->d_fitems[c], fitems[c].data(), sizeof(int2) * fitems[c].size(), hipMemcpyHostToDevice));
+    }
+    b->state_valid = false;
+    b->graph_dirty = true;
+    return upload_desc(b);
+}
+
+twdic_status twdic_batch_set_targets(twdic_batch* b, const uint8_t* const* d_tars)
+{
+    if (!b || !d_tars) return TWDIC_EINVAL;
+    for (size_t i = 0; i < b->pairs.size(); ++i) {
+        if (!d_tars[i]) continue;
+        b->pairs[i].d_tar = d_tars[i];
+        b->h_desc[i].icgn.tar = d_tars[i];
+        b->h_desc[i].fftcc.tar = d_tars[i];
+    }
+    return upload_desc(b);
+}
+
+twdic_status twdic_batch_set_references(twdic_batch* b, const uint8_t* const* d_refs)
+{
+    if (!b || !d_refs) return TWDIC_EINVAL;
+    for (size_t i = 0; i < b->pairs.size(); ++i) {
+        if (!d_refs[i]) continue;
+        b->pairs[i].d_ref = d_refs[i];
+        b->h_desc[i].icgn.ref = d_refs[i];
+        b->h_desc[i].fftcc.ref = d_refs[i];
+    }
+    b->state_valid = false;
+    return upload_desc(b);
+}
+
+twdic_status twdic_batch_run(twdic_batch* b)
+{
+    if (!b) return TWDIC_EINVAL;
+    if (!b->total) return TWDIC_OK;
+    (void)hipSetDevice(b->device);
+    if (!b->state_valid) {
+        twdic_status st = build_state(b);
+        if (st != TWDIC_OK) return st;
+    }
+    if (!b->tune.use_graph) return enqueue_frame(b, true, true);
+    if (b->graph_dirty) {
+        if (b->graph_exec) (void)hipGraphExecDestroy(b->graph_exec);
+        if (b->graph) (void)hipGraphDestroy(b->graph);
+        b->graph_exec = nullptr;
+        b->graph = nullptr;
+        TB_HIP(b, hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+        twdic_status st = enqueue_frame(b, true, true);
+        hipGraph_t g = nullptr;
+        hipError_t e = hipStreamEndCapture(b->stream, &g);
+        if (st != TWDIC_OK) {
+            if (g) (void)hipGraphDestroy(g);
+            return st;
+        }
+        if (e != hipSuccess) return bfail(b, TWDIC_EHIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+        b->graph = g;
+        TB_HIP(b, hipGraphInstantiate(&b->graph_exec, b->graph, nullptr, nullptr, 0));
+        b->graph_dirty = false;
+    }
+    TB_HIP(b, hipGraphLaunch(b->graph_exec, b->stream));
+    return TWDIC_OK;
+}
+
+twdic_status twdic_batch_run_icgn(twdic_batch* b)
+{
+    if (!b) return TWDIC_EINVAL;
+    if (!b->total) return TWDIC_OK;
+    if (!b->state_valid) {
+        twdic_status st = build_state(b);
+        if (st != TWDIC_OK) return st;
+    }
+    return enqueue_frame(b, false, true);
+}
+
+twdic_status twdic_batch_sync(twdic_batch* b)
+{
+    if (!b) return TWDIC_EINVAL;
+    TB_HIP(b, hipStreamSynchronize(b->stream));
+    return TWDIC_OK;
+}
+
+twdic_status twdic_batch_get_tuning(twdic_batch* b, twdic_batch_tuning* t)
+{
+    if (!b || !t) return TWDIC_EINVAL;
+    *t = b->tune;
+    return TWDIC_OK;
+}
+
+twdic_status twdic_batch_set_tuning(twdic_batch* b, const twdic_batch_tuning* t)
+{
+    if (!b || !t) return TWDIC_EINVAL;
+    if (t->icgn_chunk < 1 || t->icgn_chunk > 64 || t->fftcc_variant < 2 || t->fftcc_variant > 4)
+        return bfail(b, TWDIC_EINVAL, "tuning out of range");
+    b->tune = *t;
+    b->graph_dirty = true;
+    return TWDIC_OK;
+}
+
+twdic_status twdic_batch_autotune(twdic_batch* b, const char* table_path, int reps)
+{
+    if (!b || reps < 1) return TWDIC_EINVAL;
+    if (!b->total) return TWDIC_OK;
+    auto time_cfg = [&](const twdic_batch_tuning& t, float* ms) -> twdic_status {
+        twdic_status st = twdic_batch_set_tuning(b, &t);
+        if (st == TWDIC_OK) st = twdic_batch_run(b);                 // warm-up (and graph build)
+        if (st != TWDIC_OK) return st;
+        TB_HIP(b, hipEventRecord(b->ev[14], b->stream));
+        for (int r = 0; r < reps && st == TWDIC_OK; ++r) st = twdic_batch_run(b);
+        if (st != TWDIC_OK) return st;
+        TB_HIP(b, hipEventRecord(b->ev[15], b->stream));
+        TB_HIP(b, hipEventSynchronize(b->ev[15]));
+        TB_HIP(b, hipEventElapsedTime(ms, b->ev[14], b->ev[15]));
+        return TWDIC_OK;
+    };
+    twdic_batch_tuning best = b->tune;
+    float best_ms = 1e30f;
+    // stage 1: FFT-CC occupancy target (only matters with N = 64 items)
+    const int variants[3] = {2, 3, 4};
+    for (int v : variants) {
+        if (!b->nf[0] && v != best.fftcc_variant) continue;
+        twdic_batch_tuning t = best;
+        t.fftcc_variant = v;
+        float ms;
+        twdic_status st = time_cfg(t, &ms);
+        if (st != TWDIC_OK) return st;
+        if (ms < best_ms) { best_ms = ms; best = t; }
+    }
+    // stage 2: IC-GN work-queue grab size
+    const int chunks[5] = {1, 2, 4, 8, 16};
+    for (int c : chunks) {
+        twdic_batch_tuning t = best;
+        t.icgn_chunk = c;
+        float ms;
+        twdic_status st = time_cfg(t, &ms);
+        if (st != TWDIC_OK) return st;
+        if (ms < best_ms) { best_ms = ms; best = t; }
+    }
+    twdic_status st = twdic_batch_set_tuning(b, &best);
+    if (st != TWDIC_OK) return st;
+    if (table_path) {
+        std::ofstream f(table_path, std::ios::app);
+        if (!f) return bfail(b, TWDIC_EINVAL, std::string("cannot append to ") + table_path);
+        f << signature(b) << '\t' << best.icgn_chunk << ' ' << best.fftcc_variant << ' ' << best.use_graph << ' '
+          << best_ms / reps << '\n';
+    }
+    return TWDIC_OK;
+}
+
+twdic_status twdic_batch_load_tuning(twdic_batch* b, const char* table_path, int* found)
+{
+    if (!b || !table_path) return TWDIC_EINVAL;
+    if (found) *found = 0;
+    std::ifstream f(table_path);
+    if (!f) return TWDIC_OK;
+    const std::string sig = signature(b);
+    std::string line;
+    twdic_batch_tuning t = b->tune;
+    bool hit = false;
+    while (std::getline(f, line)) {        // last matching entry wins
+        auto tab = line.find('\t');
+        if (tab == std::string::npos || line.compare(0, tab, sig) != 0 || tab != sig.size()) continue;
+        std::istringstream is(line.substr(tab + 1));
+        twdic_batch_tuning c{};
+        if (is >> c.icgn_chunk >> c.fftcc_variant >> c.use_graph) {
+            t = c;
+            hit = true;
+        }
+    }
+    if (hit) {
+        twdic_status st = twdic_batch_set_tuning(b, &t);
+        if (st != TWDIC_OK) return st;
+    }
+    if (found) *found = hit ? 1 : 0;
+    return TWDIC_OK;
+}
+
+twdic_status twdic_batch_info_get(twdic_batch* b, twdic_batch_info* info)
+{
+    if (!b || !info) return TWDIC_EINVAL;
+    info->npairs = (int)b->pairs.size();
+    info->total_pois = b->total;
+    for (int c = 0; c < 3; ++c) info->fftcc_items[c] = b->nf[c];
+    info->device_bytes = b->bytes;
+    info->tuning = b->tune;
+    return TWDIC_OK;
+}
+
+twdic_status twdic_batch_record(twdic_batch* b, int slot)
+{
+    if (!b || slot < 0 || slot >= 14) return TWDIC_EINVAL;
+    TB_HIP(b, hipEventRecord(b->ev[slot], b->stream));
+    return TWDIC_OK;
+}
+
+twdic_status twdic_batch_elapsed(twdic_batch* b, int a, int c, float* ms)
+{
+    if (!b || !ms || a < 0 || c < 0 || a >= 14 || c >= 14) return TWDIC_EINVAL;
+    TB_HIP(b, hipEventSynchronize(b->ev[c]));
+    TB_HIP(b, hipEventElapsedTime(ms, b->ev[a], b->ev[c]));
+    return TWDIC_OK;
+}
+
+}  // extern "C"

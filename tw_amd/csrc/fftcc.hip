@@ -283,16 +283,19 @@ __device__ __forceinline__ int exact_corr(const uint32_t (&rw)[P / 4], const uin
     return s;
 }
 
-template <int P, int MINW>
-__global__ __launch_bounds__(16 * P, MINW) void k_fftcc_fft(FftccParams p)
+// One workgroup's work: the POIs (pa, ia) and (pb, ib); ib < 0 leaves the
+// second slot empty.  Both descriptors must share the window edge N = 4P.
+template <int P>
+__device__ __forceinline__ void fftcc_pair(FftSmem<P>& sm, const FftccParams& pa, int ia, const FftccParams& pb,
+                                           int ib)
 {
     constexpr int N = 4 * P, NN = N * N, NW = N / 16;   // waves per workgroup
-    __shared__ FftSmem<P> sm;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int line = tid >> 2, q = tid & 3, bq = ((q & 1) << 1) | (q >> 1);
-    const int poi0 = 2 * blockIdx.x;
+    const FftccParams* ps[2] = {&pa, &pb};
+    const int idx[2] = {ia, ib};
 
-    if (tid < N) sm.tw[tid] = p.tw[tid];
+    if (tid < N) sm.tw[tid] = pa.tw[tid];
 
     // -- 1. load both window pairs straight into registers (DIF layout: row
     //       `line`, columns bq*P .. bq*P+P-1), exact integer sums ---------------
@@ -301,10 +304,11 @@ __global__ __launch_bounds__(16 * P, MINW) void k_fftcc_fft(FftccParams p)
     int py[2], px[2];
 #pragma unroll
     for (int s = 0; s < 2; ++s) {
-        const int poi = poi0 + s;
+        const FftccParams& p = *ps[s];
+        const int poi = idx[s];
         valid[s] = 0;
         py[s] = px[s] = 0;
-        if (poi < p.n) {
+        if (poi >= 0) {
             py[s] = p.poi[2 * poi];
             px[s] = p.poi[2 * poi + 1];
             const int y0 = py[s] - N / 2, x0 = px[s] - N / 2;
@@ -418,6 +422,7 @@ __global__ __launch_bounds__(16 * P, MINW) void k_fftcc_fft(FftccParams p)
 #pragma unroll
     for (int s = 0; s < 2; ++s) {
         if (valid[s] == 1) {
+            const FftccParams& p = *ps[s];
             const size_t o = (size_t)(py[s] - N / 2 + line) * p.stride + (px[s] - N / 2 + bq * P);
             __builtin_memcpy(rw[s], p.ref + o, P);
             __builtin_memcpy(tw8[s], p.tar + o, P);
@@ -439,7 +444,7 @@ __global__ __launch_bounds__(16 * P, MINW) void k_fftcc_fft(FftccParams p)
 #pragma unroll
     for (int s = 0; s < 2; ++s) {
         if (valid[s] != 1) continue;                         // uniform per workgroup
-        const float thr = t[s].best - p.tol * (float)NN;
+        const float thr = t[s].best - pa.tol * (float)NN;
         int peak = t[s].idx, bestI = 0;
         if (t[s].second < thr) {
             bestI = exact_corr<P>(rw[s], tb + s * NN, line, bq, peak % N, peak / N, sm, 0);
@@ -471,19 +476,52 @@ __global__ __launch_bounds__(16 * P, MINW) void k_fftcc_fft(FftccParams p)
             }
         }
         if (tid == 0) {
+            const FftccParams& p = *ps[s];
             int iu, iv;
             wrap_peak(peak, N, iu, iv);
             const double sr = sums[s][0], st = sums[s][1];
-            p.u[poi0 + s] = (float)iu;
-            p.v[poi0 + s] = (float)iv;
-            p.zncc[poi0 + s] = (float)(((double)bestI - sr * st / (double)NN) / sqrt(nr[s] * nt[s]));
+            p.u[idx[s]] = (float)iu;
+            p.v[idx[s]] = (float)iv;
+            p.zncc[idx[s]] = (float)(((double)bestI - sr * st / (double)NN) / sqrt(nr[s] * nt[s]));
         }
     }
-    if (tid == 0 && p.status) {
+    if (tid == 0) {
 #pragma unroll
         for (int s = 0; s < 2; ++s)
-            if (valid[s]) p.status[poi0 + s] = valid[s] == 1 ? TWDIC_FFTCC_OK : valid[s] == 2 ? TWDIC_FFTCC_OUT
-                                                                                                : TWDIC_FFTCC_FLAT;
+            if (valid[s] && ps[s]->status)
+                ps[s]->status[idx[s]] = valid[s] == 1 ? TWDIC_FFTCC_OK : valid[s] == 2 ? TWDIC_FFTCC_OUT
+                                                                                       : TWDIC_FFTCC_FLAT;
+    }
+}
+
+template <int P, int MINW>
+__global__ __launch_bounds__(16 * P, MINW) void k_fftcc_fft(FftccParams p)
+{
+    __shared__ FftSmem<P> sm;
+    const int ia = 2 * blockIdx.x, ib = ia + 1 < p.n ? ia + 1 : -1;
+    fftcc_pair<P>(sm, p, ia, p, ib);
+}
+
+// Persistent form: each workgroup takes the next pair of (image pair, POI)
+// items of one window class from a device counter until the list is
+// exhausted.  Consecutive items may belong to different images.
+template <int P, int MINW>
+__global__ __launch_bounds__(16 * P, MINW) void k_batch_fftcc(const PairDesc* __restrict__ pairs,
+                                                              const int2* __restrict__ items, int total, int* counter)
+{
+    __shared__ FftSmem<P> sm;
+    __shared__ int next;
+    for (;;) {
+        if (threadIdx.x == 0) next = atomicAdd(counter, 1);
+        __syncthreads();
+        const int k = next;
+        __syncthreads();
+        if (2 * k >= total) break;
+        const int2 a = items[2 * k];
+        const int2 b = 2 * k + 1 < total ? items[2 * k + 1] : make_int2(a.x, -1);
+        const FftccParams pa = const_pairs(pairs)[__builtin_amdgcn_readfirstlane(a.x)].fftcc;
+        const FftccParams pb = const_pairs(pairs)[__builtin_amdgcn_readfirstlane(b.x)].fftcc;
+        fftcc_pair<P>(sm, pa, __builtin_amdgcn_readfirstlane(a.y), pb, __builtin_amdgcn_readfirstlane(b.y));
     }
 }
 
@@ -557,12 +595,9 @@ __device__ int exact_corr_dft(DftSmem& sm, int N, int ku, int kv)
 
 }  // namespace
 
-__global__ __launch_bounds__(kDftThreads) void k_fftcc_dft(FftccParams p)
+__device__ void fftcc_dft_poi(DftSmem& sm, const FftccParams& p, int poi)
 {
-    __shared__ DftSmem sm;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int poi = blockIdx.x;
-    if (poi >= p.n) return;
     const int N = p.N, NN = N * N, half = N >> 1;
     const int py = p.poi[2 * poi], px = p.poi[2 * poi + 1];
     const int y0 = py - half, x0 = px - half;
@@ -667,6 +702,32 @@ __global__ __launch_bounds__(kDftThreads) void k_fftcc_dft(FftccParams p)
     }
 }
 
+__global__ __launch_bounds__(kDftThreads) void k_fftcc_dft(FftccParams p)
+{
+    __shared__ DftSmem sm;
+    if ((int)blockIdx.x >= p.n) return;
+    fftcc_dft_poi(sm, p, blockIdx.x);
+}
+
+__global__ __launch_bounds__(kDftThreads) void k_batch_fftcc_dft(const PairDesc* __restrict__ pairs,
+                                                                 const int2* __restrict__ items, int total,
+                                                                 int* counter)
+{
+    __shared__ DftSmem sm;
+    __shared__ int next;
+    for (;;) {
+        if (threadIdx.x == 0) next = atomicAdd(counter, 1);
+        __syncthreads();
+        const int k = next;
+        __syncthreads();
+        if (k >= total) break;
+        const int2 it = items[k];
+        const FftccParams p = const_pairs(pairs)[__builtin_amdgcn_readfirstlane(it.x)].fftcc;
+        fftcc_dft_poi(sm, p, __builtin_amdgcn_readfirstlane(it.y));
+        __syncthreads();
+    }
+}
+
 // variant: occupancy target of the register engine (waves per SIMD the
 // compiler budgets registers for): 2 (no spills), 3 or 4.
 hipError_t launch_fftcc(const FftccParams& p, int variant, hipStream_t st)
@@ -685,6 +746,42 @@ hipError_t launch_fftcc(const FftccParams& p, int variant, hipStream_t st)
     } else
         hipLaunchKernelGGL(k_fftcc_dft, dim3(p.n), dim3(kDftThreads), 0, st, p);
     return hipGetLastError();
+}
+
+hipError_t launch_batch_fftcc(const PairDesc* pairs, const int2* items, int total, int* counter, int N, int variant,
+                              int blocks, hipStream_t st)
+{
+    if (total <= 0) return hipSuccess;
+    if (N == 64) {
+        if (variant == 2)
+            hipLaunchKernelGGL((k_batch_fftcc<16, 2>), dim3(blocks), dim3(256), 0, st, pairs, items, total, counter);
+        else if (variant == 4)
+            hipLaunchKernelGGL((k_batch_fftcc<16, 4>), dim3(blocks), dim3(256), 0, st, pairs, items, total, counter);
+        else
+            hipLaunchKernelGGL((k_batch_fftcc<16, 3>), dim3(blocks), dim3(256), 0, st, pairs, items, total, counter);
+    } else if (N == 32) {
+        hipLaunchKernelGGL((k_batch_fftcc<8, 4>), dim3(blocks), dim3(128), 0, st, pairs, items, total, counter);
+    } else {
+        hipLaunchKernelGGL(k_batch_fftcc_dft, dim3(blocks), dim3(kDftThreads), 0, st, pairs, items, total, counter);
+    }
+    return hipGetLastError();
+}
+
+int persistent_blocks_fftcc(int device, int N, int variant)
+{
+    int cus = 0, per = 0;
+    (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device);
+    hipError_t e;
+    if (N == 64)
+        e = variant == 2 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_batch_fftcc<16, 2>, 256, 0)
+            : variant == 4 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_batch_fftcc<16, 4>, 256, 0)
+                           : hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_batch_fftcc<16, 3>, 256, 0);
+    else if (N == 32)
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_batch_fftcc<8, 4>, 128, 0);
+    else
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_batch_fftcc_dft, kDftThreads, 0);
+    if (e != hipSuccess || per < 1) per = 1;
+    return (cus > 0 ? cus : 256) * per;
 }
 
 hipError_t launch_fftcc_generic(const FftccParams& p, hipStream_t st)

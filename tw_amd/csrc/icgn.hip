@@ -199,13 +199,11 @@ __device__ __forceinline__ bool subset_pass(const IcgnParams& P, int px, int py,
     return !__any(out);
 }
 
+// IC-GN for one POI by one wavefront.  poi is wave-uniform, so per-POI state
+// is read with scalar loads into SGPRs.
 template <int INTERP>
-__global__ __launch_bounds__(256) void k_icgn(IcgnParams P)
+__device__ __forceinline__ void icgn_poi(const IcgnParams& P, int poi, int lane)
 {
-    const int lane = threadIdx.x & 63;
-    // the POI index is wave-uniform: per-POI state goes to scalar registers
-    const int poi = __builtin_amdgcn_readfirstlane(xcd_block(blockIdx.x, gridDim.x) * 4 + (threadIdx.x >> 6));
-    if (poi >= P.n) return;
     const int py = __builtin_amdgcn_readfirstlane(P.poi[2 * poi]);
     const int px = __builtin_amdgcn_readfirstlane(P.poi[2 * poi + 1]);
     const int Sx = P.Sx, Sy = P.Sy;
@@ -217,7 +215,8 @@ __global__ __launch_bounds__(256) void k_icgn(IcgnParams P)
     int flag = TWDIC_ICGN_SUCCESS;
     float dpn = __builtin_nanf("");
 
-    const double* rs = P.refstate + (size_t)poi * kRefStateStride;
+    // the state block is read-only here: constant address space -> scalar loads
+    const TW_CONST_AS double* rs = (const TW_CONST_AS double*)(P.refstate + (size_t)poi * kRefStateStride);
     const int refflag = (int)rs[RS_FLAG];
     if (refflag == TWDIC_ICGN_DARK_SUBSET || refflag == TWDIC_ICGN_OUT_OF_ROI) {
         flag = refflag;
@@ -284,6 +283,53 @@ __global__ __launch_bounds__(256) void k_icgn(IcgnParams P)
         if (P.flags) P.flags[poi] = (uint8_t)flag;
         if (P.dpnorm) P.dpnorm[poi] = dpn;
     }
+}
+
+template <int INTERP>
+__global__ __launch_bounds__(256) void k_icgn(IcgnParams P)
+{
+    const int poi = __builtin_amdgcn_readfirstlane(xcd_block(blockIdx.x, gridDim.x) * 4 + (threadIdx.x >> 6));
+    if (poi >= P.n) return;
+    icgn_poi<INTERP>(P, poi, threadIdx.x & 63);
+}
+
+// Persistent form: every wavefront takes `chunk` consecutive work items
+// (image pair, POI) from a device counter until the list is exhausted; each
+// item's geometry and buffers come from its pair's descriptor.  Every wave
+// exits once the counter passes `total`.
+__global__ __launch_bounds__(256) void k_batch_icgn(const PairDesc* __restrict__ pairs, const int2* __restrict__ items,
+                                                    int total, int* counter, int chunk)
+{
+    const int lane = threadIdx.x & 63;
+    for (;;) {
+        int base = 0;
+        if (lane == 0) base = atomicAdd(counter, chunk);
+        base = __builtin_amdgcn_readfirstlane(__shfl(base, 0));
+        if (base >= total) break;
+        const int end = min(base + chunk, total);
+        for (int i = base; i < end; ++i) {
+            const int2 it = items[i];
+            const int pair = __builtin_amdgcn_readfirstlane(it.x), poi = __builtin_amdgcn_readfirstlane(it.y);
+            const IcgnParams P = const_pairs(pairs)[pair].icgn;
+            icgn_poi<TWDIC_INTERP_FUSED>(P, poi, lane);
+        }
+    }
+}
+
+int persistent_blocks_icgn(int device)
+{
+    int cus = 0, per = 0;
+    (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device);
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_batch_icgn, 256, 0) != hipSuccess || per < 1) per = 1;
+    return (cus > 0 ? cus : 256) * per;
+}
+
+hipError_t launch_batch_icgn(const PairDesc* pairs, const int2* items, int total, int* counter, int chunk,
+                             int blocks, hipStream_t st)
+{
+    if (total <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_batch_icgn, dim3(blocks), dim3(256), 0, st, pairs, items, total, counter, chunk);
+    return hipGetLastError();
 }
 
 hipError_t launch_icgn(const IcgnParams& p, int interp, hipStream_t st)

@@ -49,8 +49,8 @@ __global__ __launch_bounds__(256) void k_gradients(const uint8_t* __restrict__ i
     const uint8_t* c = img + (size_t)(y + sy0) * stride + (x + sx0);
     int l = c[-1], r = c[1], u = c[-stride], d = c[stride];
     size_t o = (size_t)y * rw + x;
-    gx[o] = 0.5f * (float)(r - l);
-    gy[o] = 0.5f * (float)(d - u);
+    if (gx) gx[o] = 0.5f * (float)(r - l);
+    if (gy) gy[o] = 0.5f * (float)(d - u);
     if (g2) g2[o] = (uint32_t)(uint16_t)(int16_t)(r - l) | ((uint32_t)(uint16_t)(int16_t)(d - u) << 16);
     if (gxy) {
         int ur = c[-stride + 1], ul = c[-stride - 1], dr = c[stride + 1], dl = c[stride - 1];
@@ -156,11 +156,8 @@ __device__ __forceinline__ double wave_sum_d(double v)
     return v;
 }
 
-__global__ __launch_bounds__(256) void k_ref_state(IcgnParams P)
+__device__ __forceinline__ void ref_state_poi(const IcgnParams& P, int poi, int lane)
 {
-    const int lane = threadIdx.x & 63;
-    const int poi = blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (poi >= P.n) return;
     const int py = P.poi[2 * poi], px = P.poi[2 * poi + 1];
     double* rs = P.refstate + (size_t)poi * kRefStateStride;
     // subsets that leave the ROI are flagged by the iteration kernel; keep the
@@ -246,9 +243,53 @@ __global__ __launch_bounds__(256) void k_ref_state(IcgnParams P)
     }
 }
 
+__global__ __launch_bounds__(256) void k_ref_state(IcgnParams P)
+{
+    const int poi = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (poi >= P.n) return;
+    ref_state_poi(P, poi, threadIdx.x & 63);
+}
+
 hipError_t launch_ref_state(const IcgnParams& p, hipStream_t st)
 {
     hipLaunchKernelGGL(k_ref_state, dim3((p.n + 3) / 4), dim3(256), 0, st, p);
+    return hipGetLastError();
+}
+
+// Persistent form over (pair, POI) work items, one item per wavefront grab.
+__global__ __launch_bounds__(256) void k_batch_ref_state(const PairDesc* __restrict__ pairs,
+                                                         const int2* __restrict__ items, int total, int* counter)
+{
+    const int lane = threadIdx.x & 63;
+    for (;;) {
+        int i = 0;
+        if (lane == 0) i = atomicAdd(counter, 1);
+        i = __builtin_amdgcn_readfirstlane(__shfl(i, 0));
+        if (i >= total) break;
+        const int2 it = items[i];
+        const IcgnParams P = const_pairs(pairs)[__builtin_amdgcn_readfirstlane(it.x)].icgn;
+        ref_state_poi(P, __builtin_amdgcn_readfirstlane(it.y), lane);
+    }
+}
+
+hipError_t launch_batch_ref_state(const PairDesc* pairs, const int2* items, int total, int* counter, int blocks,
+                                  hipStream_t st)
+{
+    if (total <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_batch_ref_state, dim3(blocks), dim3(256), 0, st, pairs, items, total, counter);
+    return hipGetLastError();
+}
+
+// Zero-fill for work counters (a kernel, so it is safe inside captured graphs).
+__global__ void k_fill_zero(int* p, int n)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0;
+}
+
+hipError_t launch_fill_zero(int* p, int n, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_fill_zero, dim3((n + 255) / 256), dim3(256), 0, st, p, n);
     return hipGetLastError();
 }
 

@@ -80,6 +80,27 @@ struct IcgnParams {
     float* dpnorm;
 };
 
+// One image pair of a batch: the same parameter blocks the single-context
+// kernels take, so the persistent kernels reuse the per-POI device code.
+struct PairDesc {
+    IcgnParams icgn;
+    FftccParams fftcc;
+};
+
+// Descriptors are never written while a batch kernel runs: reading them
+// through the constant address space lets the compiler use scalar loads and
+// keep each wave's descriptor in SGPRs.
+#if defined(__HIP_DEVICE_COMPILE__)
+#define TW_CONST_AS __attribute__((address_space(4)))
+#else
+#define TW_CONST_AS
+#endif
+typedef const TW_CONST_AS PairDesc* ConstPairDescPtr;
+__device__ __forceinline__ ConstPairDescPtr const_pairs(const PairDesc* p)
+{
+    return (ConstPairDescPtr)p;
+}
+
 // Kernel launch wrappers (return hipError_t of the launch).
 hipError_t launch_gradients(const uint8_t* img, int stride, int sx0, int sy0, int rw, int rh,
                             float* gx, float* gy, float* gxy, uint32_t* g2, hipStream_t st);
@@ -90,6 +111,17 @@ hipError_t launch_poi_grid(int32_t* poi, int start_x, int start_y, int nx, int n
 hipError_t launch_fftcc(const FftccParams& p, int variant, hipStream_t st);
 hipError_t launch_ref_state(const IcgnParams& p, hipStream_t st);
 hipError_t launch_icgn(const IcgnParams& p, int interp, hipStream_t st);
+// persistent batch kernels: `blocks` workgroups drain a device work counter
+hipError_t launch_batch_icgn(const PairDesc* pairs, const int2* items, int total, int* counter, int chunk,
+                             int blocks, hipStream_t st);
+hipError_t launch_batch_ref_state(const PairDesc* pairs, const int2* items, int total, int* counter, int blocks,
+                                  hipStream_t st);
+hipError_t launch_batch_fftcc(const PairDesc* pairs, const int2* items, int total, int* counter, int N,
+                              int variant, int blocks, hipStream_t st);
+hipError_t launch_fill_zero(int* p, int n, hipStream_t st);
+// occupancy-derived persistent grid sizes
+int persistent_blocks_icgn(int device);
+int persistent_blocks_fftcc(int device, int N, int variant);
 
 }  // namespace twdic
 
