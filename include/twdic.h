@@ -105,7 +105,10 @@ twdic_status twdic_num_poi(const twdic_config* cfg, int32_t* nx, int32_t* ny);
  * (TW_paDIC_cuFFTCC2D.cu:679-685; TW_paDIC_cuICGN2D.cu:770-793).  Copies the
  * image into the context (h_img host, or d_img device when on_device != 0;
  * stride in bytes) and marks the per-reference state stale; it is rebuilt
- * lazily by the next IC-GN call for the POI set used there. */
+ * lazily by the next IC-GN call for the POI set used there.  Both copies are
+ * ordered on this context's stream only: a device source written on another
+ * stream must be complete before the call, and the source may be reused
+ * once twdic_sync(ctx) has returned. */
 twdic_status twdic_set_reference(twdic_ctx* ctx, const uint8_t* img, int stride, int on_device);
 /* Replaces cuFFTCC2D::SetTarImg / the target upload in cuComputeFFTCC
  * (TW_paDIC_cuFFTCC2D.cu:560-563,687-693).  Host images are staged through

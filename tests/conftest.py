@@ -31,15 +31,11 @@ def orc():
     return oracle
 
 
-def pytest_collection_modifyitems(config, items):
-    # print the case name before each GPU test so a box that dies mid-run
-    # still names the test in its last output line
-    for it in items:
-        if it.get_closest_marker("gpu"):
-            it.add_marker(pytest.mark.usefixtures("_announce"))
-
-
-@pytest.fixture
-def _announce(request):
-    print(f"\n[gpu-test] {request.node.nodeid}", flush=True)
-    yield
+def pytest_runtest_setup(item):
+    # name each GPU test before any of its fixtures run, past pytest's output
+    # capture, so a run that dies or stalls mid-test still names the test in
+    # its last output line
+    if item.get_closest_marker("gpu"):
+        capman = item.config.pluginmanager.getplugin("capturemanager")
+        with capman.global_and_fixture_disabled():
+            print(f"\n[gpu-test] {item.nodeid}", flush=True)

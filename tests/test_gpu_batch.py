@@ -8,7 +8,9 @@ import pytest
 
 from tw_amd import api
 
-pytestmark = pytest.mark.gpu
+pytestmark = [pytest.mark.gpu,
+              pytest.mark.skip(reason="batched execution is switched off: its first GPU run stalled (no output "
+                                      "for 180 s) and no cause has been shown")]
 
 SPECS = [  # (w, h, S, grid, N, deformation seed)
     (256, 256, 10, 7, 32, 1),

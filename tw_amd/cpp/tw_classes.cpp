@@ -160,12 +160,16 @@ void cuFFTCC2D::ResetRefImg(const MatView& refImg)
 {
     ensureContext();
     check(twdic_set_reference(m_ctx, refImg.data, (int)refImg.step, 0), m_ctx, "ResetRefImg");
+    // the image buffer is read by other objects' streams (cuICGN2D::cuInitialize):
+    // complete the copy before returning, as the reference's cudaMemcpy does
+    check(twdic_sync(m_ctx), m_ctx, "sync");
 }
 
 void cuFFTCC2D::SetTarImg(const MatView& tarImg)
 {
     ensureContext();
     check(twdic_set_target(m_ctx, tarImg.data, (int)tarImg.step, 0), m_ctx, "SetTarImg");
+    check(twdic_sync(m_ctx), m_ctx, "sync");
 }
 
 void cuFFTCC2D::cuInitializeFFTCC(real_t*& f_d_U, real_t*& f_d_V, real_t*& f_d_ZNCC, const MatView& refImg)
@@ -343,6 +347,8 @@ void cuICGN2D::cuInitialize(uchar1* d_fRefImg)
 {
     check(twdic_set_reference(m_ctx, reinterpret_cast<const uint8_t*>(d_fRefImg), m_iImgWidth, 1), m_ctx,
           "cuInitialize");
+    // the source belongs to another object and may be overwritten once this returns
+    check(twdic_sync(m_ctx), m_ctx, "sync");
     m_isRefImgUpdated = true;
     void* p;
     check(twdic_get_buffer(m_ctx, TWDIC_BUF_RX, &p, nullptr), m_ctx, "get_buffer");
@@ -360,6 +366,7 @@ void cuICGN2D::ResetRefImg(const MatView& refImg)
         return;
     }
     check(twdic_set_reference(m_ctx, refImg.data, (int)refImg.step, 0), m_ctx, "ResetRefImg");
+    check(twdic_sync(m_ctx), m_ctx, "sync");
     m_refImg = refImg;
     m_isRefImgUpdated = true;
 }
@@ -373,6 +380,7 @@ void cuICGN2D::SetTarImg(const MatView& tarImg)
     // the reference copies into the reference buffer here (TW_paDIC_cuICGN2D.cu:805-808);
     // this class sets the target
     check(twdic_set_target(m_ctx, tarImg.data, (int)tarImg.step, 0), m_ctx, "SetTarImg");
+    check(twdic_sync(m_ctx), m_ctx, "sync");
     m_tarImg = tarImg;
 }
 
