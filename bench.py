@@ -34,6 +34,33 @@ HBM_PEAK_GBS = 8000.0
 FP32_PEAK_TFLOPS = 157.3
 
 
+def measured_traffic() -> dict:
+    """HBM bytes per launch per kernel from the committed rocprofv3 PMC
+    summary (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel factors stated
+    there), or {} when it is absent."""
+    p = ROOT / "profiles" / "traffic.json"
+    if not p.exists():
+        return {}
+    d = json.loads(p.read_text())
+    return {k: v["bytes_per_launch"] for k, v in d.get("kernels", {}).items()}
+
+
+def copy_calibration(mib: int, reps: int):
+    """A plain device-to-device copy of known size, for scaling the PMC byte
+    counters (run under rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE)."""
+    from tw_amd import api
+    ctx = api.Context(api.make_config(64, 64, 1, 1, 62, 62, 2, 2, 4, 4, 0, 0, 8))
+    nbytes = mib << 20
+    a = ctx.array(nbytes, np.uint8).fill(1)
+    b = ctx.array(nbytes, np.uint8)
+    import ctypes as C
+    for _ in range(reps):
+        ctx._check(api.lib().twdic_memcpy(ctx.h, C.c_void_p(b.ptr), C.c_void_p(a.ptr), nbytes, 2), "d2d")
+    ctx.sync()
+    ctx.close()
+    print(json.dumps({"copy_bytes": nbytes, "reps": reps}))
+
+
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -41,7 +68,10 @@ def parse():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--interp", choices=["fused", "lut"], default="fused")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-sample", type=int, default=3000, help="POIs in the CPU baseline sample")
+    ap.add_argument("--cpu-sample", type=int, default=0,
+                    help="POIs in the CPU baseline sample (0: the whole frame, about 15 s on 16 cores)")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run a MIB-sized device copy (PMC counter calibration) and exit")
     return ap.parse_args()
 
 
@@ -51,9 +81,13 @@ def cpu_baseline(case, poi, sample):
     whole-ROI precompute + FFT-CC and IC-GN on `sample` evenly spaced POIs,
     scaled to POIs/s of a full frame."""
     import oracle
+    if sample <= 0 or sample >= len(poi):
+        sample = len(poi)
     idx = np.linspace(0, len(poi) - 1, sample).astype(int)
     sp = np.ascontiguousarray(poi[idx])
-    cores = os.cpu_count() or 1
+    cores = len(os.sched_getaffinity(0))
+    if os.environ.get("OMP_NUM_THREADS", "").isdigit():
+        cores = min(cores, int(os.environ["OMP_NUM_THREADS"]))
     t0 = time.perf_counter()
     f = oracle.fftcc(case.ref, case.tar, sp, case.N, nthreads=cores)
     t1 = time.perf_counter()
@@ -71,6 +105,9 @@ def cpu_baseline(case, poi, sample):
 
 def main():
     a = parse()
+    if a.copy_calibration:
+        copy_calibration(a.copy_calibration, 4)
+        return
     if a.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {a.gpus}); the multi-GPU layout is a design "
               "note, not a measured path", file=sys.stderr)
@@ -142,19 +179,42 @@ def main():
     err = np.maximum(eu, ev_)
     kbar = float(git[ok].mean())
 
-    # roofline of the dominant kernel (IC-GN): algorithmic FP32 work per launch
+    # Roofline of each kernel from its measured launch time; the dominant
+    # (longer) one is reported as "roofline".  FLOP conventions: F_cc(N) =
+    # 3 * 2.5 N^2 log2(N^2) + 6 N (N/2 + 1) + 7 N^2 per POI (two forward and
+    # one inverse N x N transform per POI, spectrum product, moments and peak),
+    # F_it(s) = 66 s per IC-GN iteration over an s-pixel subset.
     s = (2 * case.S + 1) ** 2
+    N = case.N
+    P = case.roi[2] * case.roi[3]
     icgn_ms = float(np.mean(t_icgn))
     fft_ms = float(np.mean(t_fft))
-    flops_icgn = 66.0 * s * kbar * n                     # F_it(s) = 66 s per iteration
-    achieved_tf = flops_icgn / (icgn_ms * 1e-3) / 1e12
-    # algorithmic HBM bytes per IC-GN launch: target image + Rx/Ry + per-POI
-    # state (288 B) + POI (8 B) + results (u, v, p6, iters, flag = 41 B)
-    P = case.roi[2] * case.roi[3]
-    bytes_icgn = W * H + 8 * P + n * (288 + 8 + 41)
-    if interp == api.INTERP_LUT:
-        bytes_icgn += 64 * P
-    hbm_gbs = bytes_icgn / (icgn_ms * 1e-3) / 1e9
+    f_cc = 3 * 2.5 * N * N * np.log2(N * N) + 6 * N * (N // 2 + 1) + 7 * N * N
+    kern = {
+        "k_fftcc": {"ms": fft_ms, "flops": f_cc * n,
+                    # both images read once, POI in, u/v/ZNCC out
+                    "bytes": 2 * W * H + n * (8 + 12)},
+        "k_icgn": {"ms": icgn_ms, "flops": 66.0 * s * kbar * n,
+                   # target image, packed reference gradients (4 B/px), per-POI
+                   # reference state read (42 doubles), POI + guess in,
+                   # u, v, p6, iterations and flag out
+                   "bytes": W * H + 4 * P + n * (42 * 8 + 8 + 8 + 41) + (64 * P if interp == api.INTERP_LUT else 0)},
+    }
+    traffic = measured_traffic()
+    for k, d in kern.items():
+        d["tflops"] = d["flops"] / (d["ms"] * 1e-3) / 1e12
+        d["gbs"] = d["bytes"] / (d["ms"] * 1e-3) / 1e9
+        d["traffic"] = traffic.get(k)
+    dom = max(kern, key=lambda k: kern[k]["ms"])
+    D = kern[dom]
+
+    def roof(name, d):
+        return {"bound": "mfma", "pipe": "fp32 VALU (no MFMA: no dense contraction on this path)",
+                "kernel": name, "achieved": d["tflops"], "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+                "frac": d["tflops"] / FP32_PEAK_TFLOPS, "traffic": d["traffic"],
+                "avg_launch_ms": d["ms"], "algorithmic_flops_per_launch": d["flops"],
+                "hbm": {"achieved": d["gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": d["gbs"] / HBM_PEAK_GBS,
+                        "algorithmic_bytes_per_launch": d["bytes"]}}
 
     out = {
         "metric": "IC-GN POIs/s and frames/s at 1 GPU; p50/p99 frame latency; subpixel err px",
@@ -181,12 +241,8 @@ def main():
         "flags": np.bincount(gfl, minlength=5).tolist(),
         "subpixel_err_px": {"median": float(np.median(err)), "p99": float(np.percentile(err, 99)),
                             "max": float(err.max()), "vs": "analytic affine field"},
-        "roofline": {"bound": "mfma", "pipe": "fp32 VALU (no MFMA: no dense contraction on this path)",
-                     "kernel": "k_icgn", "achieved": achieved_tf, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved_tf / FP32_PEAK_TFLOPS, "traffic": None,
-                     "algorithmic_flops_per_launch": flops_icgn,
-                     "hbm": {"achieved": hbm_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                             "frac": hbm_gbs / HBM_PEAK_GBS, "algorithmic_bytes_per_launch": bytes_icgn}},
+        "roofline": roof(dom, D),
+        "kernels": {k: roof(k, d) for k, d in kern.items()},
         "synth_s": t_gen,
     }
     if not a.no_cpu_baseline:
