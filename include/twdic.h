@@ -185,9 +185,9 @@ twdic_status twdic_precompute_target(twdic_ctx* ctx);
 
 twdic_status twdic_malloc(twdic_ctx* ctx, void** d_ptr, size_t bytes);
 twdic_status twdic_free(twdic_ctx* ctx, void* d_ptr);
-/* Asynchronous on the context stream when h_ptr is pinned; synchronous
- * otherwise.  kind: 0 = host to device, 1 = device to host, 2 = device to
- * device. */
+/* kind: 0 = host to device, 1 = device to host, 2 = device to device.
+ * Host transfers complete before the call returns; device-to-device copies
+ * are asynchronous on the context stream. */
 twdic_status twdic_memcpy(twdic_ctx* ctx, void* dst, const void* src, size_t bytes, int kind);
 twdic_status twdic_memset(twdic_ctx* ctx, void* d_ptr, int value, size_t bytes);
 

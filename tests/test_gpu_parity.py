@@ -22,6 +22,7 @@ import pytest
 
 import oracle
 from tests import cases
+from tests.gpu_helpers import check_icgn, ctx_for, run_fftcc, run_icgn
 from tw_amd import api
 
 pytestmark = pytest.mark.gpu
@@ -32,44 +33,10 @@ def c512():
     return cases.speckle_512()
 
 
-def _ctx(case, **kw):
-    ctx = api.Context(case.config(**kw))
-    ctx.set_reference(case.ref)
-    ctx.set_target(case.tar)
-    return ctx
-
-
-def _run_fftcc(ctx, poi):
-    n = len(poi)
-    dpoi = ctx.array(poi.shape, np.int32).upload(poi)
-    u = ctx.array(n, np.float32).fill(0)
-    v = ctx.array(n, np.float32).fill(0)
-    z = ctx.array(n, np.float32).fill(0)
-    st = ctx.array(n, np.uint8).fill(255)
-    ctx.fftcc(dpoi, n, u, v, z, st)
-    ctx.sync()
-    return dict(u=u.download(), v=v.download(), zncc=z.download(), status=st.download())
-
-
-def _run_icgn(ctx, poi, u0, v0):
-    n = len(poi)
-    dpoi = ctx.array(poi.shape, np.int32).upload(poi)
-    u = ctx.array(n, np.float32).upload(u0.astype(np.float32))
-    v = ctx.array(n, np.float32).upload(v0.astype(np.float32))
-    p6 = ctx.array((n, 6), np.float32).fill(0)
-    it = ctx.array(n, np.int32).upload(np.full(n, -1, np.int32))
-    fl = ctx.array(n, np.uint8).fill(255)
-    dn = ctx.array(n, np.float32).fill(0)
-    ctx.icgn(dpoi, n, u, v, p6, it, fl, dn)
-    ctx.sync()
-    return dict(u=u.download(), v=v.download(), p6=p6.download(), iters=it.download(), flags=fl.download(),
-                dpnorm=dn.download())
-
-
 # ------------------------------------------------------------------ precompute
 def test_gradients_and_lut_bit_exact(c512):
     x, y, w, h = c512.roi
-    with _ctx(c512) as ctx:
+    with ctx_for(c512) as ctx:
         ctx.precompute_target()
         ctx.sync()
         rx = ctx.download_buffer(api.BUF_RX, (h, w), np.float32)
@@ -91,8 +58,8 @@ def test_fftcc_peaks_exact(c512, N):
     """N = 32: radix-4 register engine; N = 30 = 2S: the reference's own window
     (direct-DFT engine); 20, 24: further generic sizes."""
     poi = c512.poi()
-    with _ctx(c512, N=N) as ctx:
-        g = _run_fftcc(ctx, poi)
+    with ctx_for(c512, N=N) as ctx:
+        g = run_fftcc(ctx, poi)
     ex = oracle.fftcc_exact(c512.ref, c512.tar, poi, N)
     f32 = oracle.fftcc(c512.ref, c512.tar, poi, N)
     ok = ex["status"] == oracle.FFTCC_OK
@@ -117,10 +84,10 @@ def test_fftcc_shift_recovery():
     tar = np.roll(img, (3, -5), axis=(0, 1))          # T(x - 5, y + 3) = R(x, y): u = -5, v = 3
     case = cases.Case("roll", 256, 16, 40, 32, (1, 1, 254, 254), 20, img, tar, None)
     poi = case.poi()
-    with _ctx(case) as ctx:
-        g = _run_fftcc(ctx, poi)
+    with ctx_for(case) as ctx:
+        g = run_fftcc(ctx, poi)
         ctx.set_target(img)
-        same = _run_fftcc(ctx, poi)
+        same = run_fftcc(ctx, poi)
     assert np.all(g["u"] == -5) and np.all(g["v"] == 3)
     ex = oracle.fftcc_exact(img, tar, poi, 32)
     assert np.max(np.abs(g["zncc"] - ex["zncc"])) <= 1e-6
@@ -130,33 +97,6 @@ def test_fftcc_shift_recovery():
 
 
 # ----------------------------------------------------------------------- IC-GN
-def _check_icgn(g, o, delta_p=0.001, utol=1e-4, gtol=1e-6, band=0.05):
-    """Parity with the oracle o.  Iteration counts may differ only where a
-    convergence decision sat inside the FP32 noise band of the loop test: the
-    side that stopped first saw |dp| within `band` (relative) of delta_p.  The
-    FP32 CPU path carries ~1 % rounding error in |dp| at that point (measured
-    against the FP64 oracle), so this is the reference's own noise."""
-    assert np.array_equal(g["flags"], o["flags"]), np.nonzero(g["flags"] != o["flags"])
-    same = g["iters"] == o["iters"]
-    flipped = np.nonzero(~same)[0]
-    for i in flipped:
-        if g["iters"][i] < o["iters"][i] and g.get("dpnorm") is not None:
-            stopper = g["dpnorm"][i]
-        elif g["iters"][i] > o["iters"][i]:
-            stopper = o["dpnorm"][i]
-        else:
-            continue        # GPU stopped first but its |dp| was not returned
-        assert abs(stopper / delta_p - 1.0) < band, (i, g["iters"][i], o["iters"][i], stopper)
-    assert len(flipped) <= max(2, int(0.002 * len(same)))
-    ok = same & (o["flags"] == oracle.SUCCESS)
-    du = max(np.max(np.abs(g["u"][ok] - o["u"][ok])), np.max(np.abs(g["v"][ok] - o["v"][ok])))
-    dg = np.max(np.abs(g["p6"][ok][:, [1, 2, 4, 5]] - o["p6"][ok][:, [1, 2, 4, 5]]))
-    print(f"  max |du|,|dv| = {du:.3g} (bar {utol:g}); max |d grad| = {dg:.3g} (bar {gtol:g}); "
-          f"iteration flips {len(flipped)}")
-    assert du <= utol and dg <= gtol
-    return len(flipped)
-
-
 @pytest.mark.parametrize("interp", [api.INTERP_LUT, api.INTERP_FUSED])
 def test_icgn_parity(c512, interp):
     poi = c512.poi()
@@ -164,16 +104,16 @@ def test_icgn_parity(c512, interp):
     o = oracle.icgn(c512.ref, c512.tar, c512.roi, 15, 15, poi, f["u"], f["v"])
     o64 = oracle.icgn(c512.ref, c512.tar, c512.roi, 15, 15, poi, f["u"].astype(np.float64),
                       f["v"].astype(np.float64), precision="f64")
-    with _ctx(c512, interp=interp) as ctx:
-        g = _run_icgn(ctx, poi, f["u"], f["v"])
+    with ctx_for(c512, interp=interp) as ctx:
+        g = run_icgn(ctx, poi, f["u"], f["v"])
     # against the FP32 CPU path: u, v within 1e-4 px; displacement gradients
     # within 3e-6, twice the FP32 path's own deviation from FP64 (1.4e-6 max on
     # this workload, tests/test_oracle.py measures it)
-    _check_icgn(g, o, gtol=3e-6)
+    check_icgn(g, o, gtol=3e-6)
     # against the FP64 oracle: u, v within 5e-5 px, gradients within 2e-6 (the
     # FP32 warp arithmetic shared with the reference path sets this floor:
     # the FP32 CPU path itself sits 1.4e-6 from FP64 on this workload)
-    n64 = _check_icgn(g, o64, utol=5e-5, gtol=2e-6)
+    n64 = check_icgn(g, o64, utol=5e-5, gtol=2e-6)
     assert n64 <= 2
     # sub-pixel accuracy against the analytic translation
     ok = o["flags"] == 0
@@ -189,14 +129,14 @@ def test_icgn_identity_and_integer_shift():
     case = cases.Case("shift", 300, 10, 20, 32, (1, 1, 298, 298), 12, base, tar, None)
     poi = case.poi()
     n = len(poi)
-    with _ctx(case) as ctx:
+    with ctx_for(case) as ctx:
         # T = R: first update is exactly zero, converged after 1 iteration
         ctx.set_target(base)
-        g = _run_icgn(ctx, poi, np.zeros(n), np.zeros(n))
+        g = run_icgn(ctx, poi, np.zeros(n), np.zeros(n))
         assert np.all(g["flags"] == 0) and np.all(g["iters"] == 1)
         assert np.all(g["u"] == 0) and np.all(g["v"] == 0) and np.all(g["p6"] == 0)
         ctx.set_target(tar)
-        g = _run_icgn(ctx, poi, np.full(n, -3.0), np.full(n, 2.0))
+        g = run_icgn(ctx, poi, np.full(n, -3.0), np.full(n, 2.0))
         assert np.all(g["flags"] == 0) and np.all(g["iters"] == 1)
         assert np.all(g["u"] == -3) and np.all(g["v"] == 2)
 
@@ -215,8 +155,8 @@ def test_icgn_flags():
     v0 = np.zeros(4, np.float32)
     case = cases.Case("flags", 256, 10, 10, 20, (1, 1, 254, 254), 0, ref, tar, None)
     o = oracle.icgn(ref, tar, case.roi, 10, 10, poi, u0, v0)
-    with _ctx(case) as ctx:
-        g = _run_icgn(ctx, poi, u0, v0)
+    with ctx_for(case) as ctx:
+        g = run_icgn(ctx, poi, u0, v0)
     assert list(o["flags"]) == [oracle.DARK_SUBSET, oracle.SINGULAR_HESSIAN, oracle.SUCCESS, oracle.OUT_OF_ROI]
     assert np.array_equal(g["flags"], o["flags"])
     assert np.array_equal(g["iters"], o["iters"])
@@ -228,7 +168,7 @@ def test_frame_end_to_end(c512):
     oracle IC-GN."""
     poi = c512.poi()
     n = len(poi)
-    with _ctx(c512) as ctx:
+    with ctx_for(c512) as ctx:
         dpoi = ctx.poi_grid()
         assert np.array_equal(dpoi.download(), poi)
         u = ctx.array(n, np.float32).fill(0)
@@ -243,4 +183,4 @@ def test_frame_end_to_end(c512):
     ex = oracle.fftcc_exact(c512.ref, c512.tar, poi, 32)
     o = oracle.icgn(c512.ref, c512.tar, c512.roi, 15, 15, poi, ex["u"].astype(np.float32),
                     ex["v"].astype(np.float32))
-    _check_icgn(g, o, gtol=3e-6)
+    check_icgn(g, o, gtol=3e-6)
