@@ -148,8 +148,8 @@ __device__ __forceinline__ int xcd_block(int b, int nb)
 // target pixels (:610-637); otherwise bicubic (:757-792).  Returns false when
 // a warped pixel leaves the ROI.  Accumulates sum T, sum T^2 and the six
 // sum SD_k T / 2 (the gradient plane stores 2 Rx, 2 Ry) in FP64.
-template <int INTERP, bool INT>
-__device__ __forceinline__ bool subset_pass(const IcgnParams& P, int px, int py, const float Wm[6], int lane,
+template <int INTERP, bool INT, bool CHECK>
+__device__ __forceinline__ bool subset_loop(const IcgnParams& P, int px, int py, const float Wm[6], int lane,
                                             double acc[8])
 {
     const int Sx = P.Sx, Sy = P.Sy, Wsub = P.Wsub, s = P.s;
@@ -164,19 +164,21 @@ __device__ __forceinline__ bool subset_pass(const IcgnParams& P, int px, int py,
         const float wx = warp_coord(px, Wm[0], Wm[1], Wm[2], dx, dy);
         const float wy = warp_coord(py, Wm[3], Wm[4], Wm[5], dx, dy);
         const int ix = (int)wx, iy = (int)wy;          // truncation, as int()
-        const bool in = ix >= sx0 && iy >= sy0 && ix < sx0 + rw && iy < sy0 + rh;
-        out |= !in;
         float T = 0.f;
-        if (in) {
-            if (INT) {
-                T = (float)P.tar[(uint32_t)(iy * P.stride + ix)];                  // :629
-            } else if (ix != sx0 + rw - 1 && iy != sy0 + rh - 1) {   // the CPU LUT's last row/column is zero
-                const float tx = wx - (float)ix, ty = wy - (float)iy;            // :771-772
-                if (INTERP == TWDIC_INTERP_LUT)
-                    T = interp_lut(P.lut, (uint32_t)((iy - sy0) * rw + (ix - sx0)), tx, ty);
-                else
-                    T = interp_fused(P.tar, P.stride, ix, iy, tx, ty);
-            }
+        bool in = true, interior = true;
+        if (CHECK) {
+            in = ix >= sx0 && iy >= sy0 && ix < sx0 + rw && iy < sy0 + rh;
+            interior = in && ix != sx0 + rw - 1 && iy != sy0 + rh - 1;   // the CPU LUT's last row/column is zero
+            out |= !in;
+        }
+        if (INT) {
+            if (in) T = (float)P.tar[(uint32_t)(iy * P.stride + ix)];              // :629
+        } else if (interior) {
+            const float tx = wx - (float)ix, ty = wy - (float)iy;                // :771-772
+            if (INTERP == TWDIC_INTERP_LUT)
+                T = interp_lut(P.lut, (uint32_t)((iy - sy0) * rw + (ix - sx0)), tx, ty);
+            else
+                T = interp_fused(P.tar, P.stride, ix, iy, tx, ty);
         }
         const uint32_t g = g2[l * rw + m];
         const double gx = (double)(int16_t)(g & 0xffffu), gy = (double)(int16_t)(g >> 16);
@@ -196,7 +198,33 @@ __device__ __forceinline__ bool subset_pass(const IcgnParams& P, int px, int py,
     acc[0] = st; acc[1] = stt;
     acc[2] = 0.5 * a0; acc[3] = 0.5 * a1; acc[4] = 0.5 * a2;
     acc[5] = 0.5 * b0; acc[6] = 0.5 * b1; acc[7] = 0.5 * b2;
-    return !__any(out);
+    return CHECK ? !__any(out) : true;
+}
+
+// One pass over the subset at the current warp.  INT: first pass at integer
+// target pixels (:610-637); otherwise bicubic (:757-792).  Returns false when
+// a warped pixel leaves the ROI.  Accumulates sum T, sum T^2 and the six
+// sum SD_k T / 2 (the gradient plane stores 2 Rx, 2 Ry) in FP64.
+// The warp is affine, so over the subset its coordinates are extremal at the
+// four corners.  When every corner lands at least one pixel inside the
+// region where no per-pixel test can fire (the margin covers FP32 rounding
+// of the per-pixel warp, which is below 1e-3 px), the pass runs without the
+// per-pixel tests; the results are identical either way.
+template <int INTERP, bool INT>
+__device__ __forceinline__ bool subset_pass(const IcgnParams& P, int px, int py, const float Wm[6], int lane,
+                                            double acc[8])
+{
+    const float ex = (float)P.Sx, ey = (float)P.Sy;
+    const float x0 = warp_coord(px, Wm[0], Wm[1], Wm[2], -ex, -ey), x1 = warp_coord(px, Wm[0], Wm[1], Wm[2], ex, -ey);
+    const float x2 = warp_coord(px, Wm[0], Wm[1], Wm[2], -ex, ey), x3 = warp_coord(px, Wm[0], Wm[1], Wm[2], ex, ey);
+    const float y0 = warp_coord(py, Wm[3], Wm[4], Wm[5], -ex, -ey), y1 = warp_coord(py, Wm[3], Wm[4], Wm[5], ex, -ey);
+    const float y2 = warp_coord(py, Wm[3], Wm[4], Wm[5], -ex, ey), y3 = warp_coord(py, Wm[3], Wm[4], Wm[5], ex, ey);
+    const bool safe = fminf(fminf(x0, x1), fminf(x2, x3)) >= (float)(P.sx0 + 1) &&
+                      fmaxf(fmaxf(x0, x1), fmaxf(x2, x3)) < (float)(P.sx0 + P.rw - 2) &&
+                      fminf(fminf(y0, y1), fminf(y2, y3)) >= (float)(P.sy0 + 1) &&
+                      fmaxf(fmaxf(y0, y1), fmaxf(y2, y3)) < (float)(P.sy0 + P.rh - 2);
+    if (safe) return subset_loop<INTERP, INT, false>(P, px, py, Wm, lane, acc);
+    return subset_loop<INTERP, INT, true>(P, px, py, Wm, lane, acc);
 }
 
 // IC-GN for one POI by one wavefront.  poi is wave-uniform, so per-POI state
@@ -285,8 +313,10 @@ __device__ __forceinline__ void icgn_poi(const IcgnParams& P, int poi, int lane)
     }
 }
 
-template <int INTERP>
-__global__ __launch_bounds__(256) void k_icgn(IcgnParams P)
+// MINW: occupancy target (waves per SIMD) the register budget is fitted to;
+// 0 leaves the choice to the compiler.
+template <int INTERP, int MINW>
+__global__ __launch_bounds__(256, MINW > 0 ? MINW : 1) void k_icgn(IcgnParams P)
 {
     const int poi = __builtin_amdgcn_readfirstlane(xcd_block(blockIdx.x, gridDim.x) * 4 + (threadIdx.x >> 6));
     if (poi >= P.n) return;
@@ -332,14 +362,25 @@ hipError_t launch_batch_icgn(const PairDesc* pairs, const int2* items, int total
     return hipGetLastError();
 }
 
-hipError_t launch_icgn(const IcgnParams& p, int interp, hipStream_t st)
+template <int INTERP>
+static void launch_icgn_t(const IcgnParams& p, int variant, dim3 grid, hipStream_t st)
+{
+    switch (variant) {
+    case 5: hipLaunchKernelGGL((k_icgn<INTERP, 5>), grid, dim3(256), 0, st, p); break;
+    case 6: hipLaunchKernelGGL((k_icgn<INTERP, 6>), grid, dim3(256), 0, st, p); break;
+    case 7: hipLaunchKernelGGL((k_icgn<INTERP, 7>), grid, dim3(256), 0, st, p); break;
+    default: hipLaunchKernelGGL((k_icgn<INTERP, 0>), grid, dim3(256), 0, st, p); break;
+    }
+}
+
+hipError_t launch_icgn(const IcgnParams& p, int interp, int variant, hipStream_t st)
 {
     if (p.n <= 0) return hipSuccess;
     dim3 grid((p.n + 3) / 4);
     if (interp == TWDIC_INTERP_FUSED)
-        hipLaunchKernelGGL(k_icgn<TWDIC_INTERP_FUSED>, grid, dim3(256), 0, st, p);
+        launch_icgn_t<TWDIC_INTERP_FUSED>(p, variant, grid, st);
     else
-        hipLaunchKernelGGL(k_icgn<TWDIC_INTERP_LUT>, grid, dim3(256), 0, st, p);
+        launch_icgn_t<TWDIC_INTERP_LUT>(p, variant, grid, st);
     return hipGetLastError();
 }
 

@@ -153,6 +153,7 @@ twdic_status twdic_create(const twdic_config* cfg, int device, twdic_ctx** out)
     num_poi(*cfg, &c->nx, &c->ny);
     c->dstride = cfg->img_w;
     if (const char* v = std::getenv("TWDIC_FFTCC_VARIANT")) c->fftcc_variant = std::atoi(v);
+    if (const char* v = std::getenv("TWDIC_ICGN_VARIANT")) c->icgn_variant = std::atoi(v);
     twdic_status st = TWDIC_OK;
     auto bail = [&](hipError_t e, const char* w) {
         hip_fail(c, e, w);
@@ -321,7 +322,7 @@ twdic_status twdic_icgn(twdic_ctx* c, const int32_t* d_poi, int n, float* d_u, f
         c->refstate_poi = d_poi;
         c->refstate_n = n;
     }
-    TW_HIP(c, launch_icgn(p, g.interp, c->stream));
+    TW_HIP(c, launch_icgn(p, g.interp, c->icgn_variant, c->stream));
     return TWDIC_OK;
 }
 

@@ -110,7 +110,7 @@ hipError_t launch_poi_grid(int32_t* poi, int start_x, int start_y, int nx, int n
                            int sx, int sy, int gx, int gy, hipStream_t st);
 hipError_t launch_fftcc(const FftccParams& p, int variant, hipStream_t st);
 hipError_t launch_ref_state(const IcgnParams& p, hipStream_t st);
-hipError_t launch_icgn(const IcgnParams& p, int interp, hipStream_t st);
+hipError_t launch_icgn(const IcgnParams& p, int interp, int variant, hipStream_t st);
 // persistent batch kernels: `blocks` workgroups drain a device work counter
 hipError_t launch_batch_icgn(const PairDesc* pairs, const int2* items, int total, int* counter, int chunk,
                              int blocks, hipStream_t st);
@@ -152,6 +152,7 @@ struct twdic_ctx {
     const int32_t* refstate_poi = nullptr;
     int refstate_n = -1;
     int fftcc_variant = 3;            // tuned launch variant of the FFT-CC engine
+    int icgn_variant = 0;             // occupancy target of the IC-GN kernel (0: compiler's choice)
     uint8_t* h_stage = nullptr;       // pinned staging for host images
     size_t h_stage_bytes = 0;
     hipEvent_t ev[16] = {};
