@@ -159,6 +159,7 @@ __device__ __forceinline__ bool subset_loop(const IcgnParams& P, int px, int py,
     const uint32_t* g2 = P.g2 + (size_t)(py - Sy - sy0) * rw + (px - Sx - sx0);
     double st = 0.0, stt = 0.0, a0 = 0.0, a1 = 0.0, a2 = 0.0, b0 = 0.0, b1 = 0.0, b2 = 0.0;
     bool out = false;
+#pragma unroll 2
     for (int p = lane; p < s; p += 64) {
         const float dx = (float)(m - Sx), dy = (float)(l - Sy);
         const float wx = warp_coord(px, Wm[0], Wm[1], Wm[2], dx, dy);

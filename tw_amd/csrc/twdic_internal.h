@@ -152,7 +152,7 @@ struct twdic_ctx {
     const int32_t* refstate_poi = nullptr;
     int refstate_n = -1;
     int fftcc_variant = 3;            // tuned launch variant of the FFT-CC engine
-    int icgn_variant = 0;             // occupancy target of the IC-GN kernel (0: compiler's choice)
+    int icgn_variant = 5;             // occupancy target of the IC-GN kernel (0: compiler's choice)
     uint8_t* h_stage = nullptr;       // pinned staging for host images
     size_t h_stage_bytes = 0;
     hipEvent_t ev[16] = {};
