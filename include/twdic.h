@@ -166,6 +166,30 @@ twdic_status twdic_frame(twdic_ctx* ctx, const int32_t* d_poi_yx, int n,
                          float* d_u, float* d_v, float* d_zncc, float* d_p6,
                          int32_t* d_iters, uint8_t* d_flags);
 
+/* ------------------------------------------------------------ POI tracking */
+
+/* The tracking step TW_Core runs around a reference reset
+ * (FFTCCTWorkerThread::processFrameFFTCC_ICGN,
+ * TW/TW_Core/fftcctworkerthread.cpp:408-438,476-481).  All three are
+ * asynchronous on ctx's stream; with ctx == NULL they run on the default
+ * stream and complete before returning (the reference's wrappers launch on
+ * the legacy default stream).  int() truncates toward zero, as the reference. */
+
+/* Replaces cuUpdatePOIpos / updatePOIpos_Kernel (TW/TW_Core/cuda_utils.cu:
+ * 174-186,271-284): d_poi_yx[i] += (int(v[i]), int(u[i])) in place.  When
+ * d_poi_yx is the POI set ctx's per-POI reference state was built for, that
+ * state is marked stale. */
+twdic_status twdic_track_update_poi(twdic_ctx* ctx, int32_t* d_poi_yx, int n, const float* d_u, const float* d_v);
+/* Replaces cuAccumulatePOI / accumulatePOI_Kernel (cuda_utils.cu:188-201,
+ * 286-301): d_out_yx[i] = d_poi_yx[i] + (int(v[i]), int(u[i])), the POIs'
+ * integer positions in the current target. */
+twdic_status twdic_track_current_poi(twdic_ctx* ctx, const int32_t* d_poi_yx, int n, const float* d_u,
+                                     const float* d_v, int32_t* d_out_yx);
+/* Replaces cuAccumulateUV / accumulateUV_Kernel (cuda_utils.cu:225-239,
+ * 323-338): d_acc_u[i] += d_u[i], d_acc_v[i] += d_v[i]. */
+twdic_status twdic_track_accumulate_uv(twdic_ctx* ctx, int n, const float* d_u, const float* d_v, float* d_acc_u,
+                                       float* d_acc_v);
+
 /* ------------------------------------------------ precompute (inspection) */
 
 /* Device buffers the reference exposes as public members of

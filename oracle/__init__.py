@@ -53,6 +53,9 @@ def lib() -> C.CDLL:
             "orc_lut_eval_f32": ([vp, f, f], f),
             "orc_warp_update_f32": ([vp, vp], i),
             "orc_warp_update_f64": ([vp, vp], i),
+            "orc_track_update_poi": ([vp, i, vp, vp], None),
+            "orc_track_current_poi": ([vp, i, vp, vp, vp], None),
+            "orc_track_accumulate_uv": ([i, vp, vp, vp, vp], None),
         }
         for name, (a, r) in sig.items():
             fn = getattr(L, name)
@@ -188,3 +191,31 @@ def warp_update(p, dp, precision="f64"):
 def lut_eval(cell16: np.ndarray, tx: float, ty: float) -> float:
     c = np.ascontiguousarray(cell16, np.float32)
     return float(lib().orc_lut_eval_f32(_p(c), tx, ty))
+
+
+def track_update_poi(poi_yx, u, v) -> np.ndarray:
+    """Copy of poi_yx moved by (int(v), int(u)) (TW_Core updatePOIpos_Kernel)."""
+    out = np.array(poi_yx, np.int32, copy=True, order="C")
+    u = np.ascontiguousarray(u, np.float32)
+    v = np.ascontiguousarray(v, np.float32)
+    lib().orc_track_update_poi(_p(out), len(out), _p(u), _p(v))
+    return out
+
+
+def track_current_poi(poi_yx, u, v) -> np.ndarray:
+    poi = np.ascontiguousarray(poi_yx, np.int32)
+    u = np.ascontiguousarray(u, np.float32)
+    v = np.ascontiguousarray(v, np.float32)
+    out = np.empty_like(poi)
+    lib().orc_track_current_poi(_p(poi), len(poi), _p(u), _p(v), _p(out))
+    return out
+
+
+def track_accumulate_uv(acc_u, acc_v, u, v):
+    """Returns (acc_u + u, acc_v + v) computed as the reference does, in FP32."""
+    au = np.array(acc_u, np.float32, copy=True)
+    av = np.array(acc_v, np.float32, copy=True)
+    u = np.ascontiguousarray(u, np.float32)
+    v = np.ascontiguousarray(v, np.float32)
+    lib().orc_track_accumulate_uv(len(au), _p(u), _p(v), _p(au), _p(av))
+    return au, av

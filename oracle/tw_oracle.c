@@ -139,3 +139,30 @@ int orc_fftcc_exact(const uint8_t* ref, const uint8_t* tar, int stride,
     }
     return 0;
 }
+
+/* F1: TW/TW_Core/cuda_utils.cu:174-186 (updatePOIpos_Kernel): y += int(v), x += int(u). */
+void orc_track_update_poi(int32_t* poi_yx, int n, const float* u, const float* v)
+{
+    for (int i = 0; i < n; ++i) {
+        poi_yx[i * 2 + 0] += (int)v[i];
+        poi_yx[i * 2 + 1] += (int)u[i];
+    }
+}
+
+/* cuda_utils.cu:188-201 (accumulatePOI_Kernel). */
+void orc_track_current_poi(const int32_t* poi_yx, int n, const float* u, const float* v, int32_t* out_yx)
+{
+    for (int i = 0; i < n; ++i) {
+        out_yx[i * 2 + 0] = poi_yx[i * 2 + 0] + (int)v[i];
+        out_yx[i * 2 + 1] = poi_yx[i * 2 + 1] + (int)u[i];
+    }
+}
+
+/* cuda_utils.cu:225-239 (accumulateUV_Kernel): the first pair accumulates. */
+void orc_track_accumulate_uv(int n, const float* u, const float* v, float* acc_u, float* acc_v)
+{
+    for (int i = 0; i < n; ++i) {
+        acc_u[i] += u[i];
+        acc_v[i] += v[i];
+    }
+}

@@ -141,6 +141,15 @@ int orc_warp_update_f64(double* p, const double* dp);
  * (TW_paDIC_ICGN2D_CPU.cpp:775-784). */
 float orc_lut_eval_f32(const float* cell16, float tx, float ty);
 
+/* F1: POI tracking around a reference reset (TW/TW_Core/cuda_utils.cu), in
+ * the caller's order of use at fftcctworkerthread.cpp:419-437,476-481.
+ *   update_poi   (updatePOIpos_Kernel, :174-186): poi[i] += (int(v), int(u))
+ *   current_poi  (accumulatePOI_Kernel, :188-201): out[i] = poi[i] + (int(v), int(u))
+ *   accumulate_uv(accumulateUV_Kernel, :225-239): acc_u += u, acc_v += v */
+void orc_track_update_poi(int32_t* poi_yx, int n, const float* u, const float* v);
+void orc_track_current_poi(const int32_t* poi_yx, int n, const float* u, const float* v, int32_t* out_yx);
+void orc_track_accumulate_uv(int n, const float* u, const float* v, float* acc_u, float* acc_v);
+
 #ifdef __cplusplus
 }
 #endif

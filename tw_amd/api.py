@@ -85,6 +85,9 @@ _SIGS = {
     "twdic_free": ([_VP, _VP], _I),
     "twdic_memcpy": ([_VP, _VP, _VP, _SZ, _I], _I),
     "twdic_memset": ([_VP, _VP, _I, _SZ], _I),
+    "twdic_track_update_poi": ([_VP, _VP, _I, _VP, _VP], _I),
+    "twdic_track_current_poi": ([_VP, _VP, _I, _VP, _VP, _VP], _I),
+    "twdic_track_accumulate_uv": ([_VP, _I, _VP, _VP, _VP, _VP], _I),
     "twdic_event_record": ([_VP, _I], _I),
     "twdic_event_elapsed": ([_VP, _I, _I, C.POINTER(C.c_float)], _I),
     "twdic_batch_create": ([_I, C.POINTER(_VP)], _I),
@@ -299,6 +302,16 @@ class Context:
         assert out.nbytes <= b
         self._check(lib().twdic_memcpy(self.h, _ptr(out), p, out.nbytes, 1), "memcpy d2h")
         return out
+
+    def track_update_poi(self, poi: DeviceArray, n: int, u: DeviceArray, v: DeviceArray):
+        self._check(lib().twdic_track_update_poi(self.h, poi.ptr, n, u.ptr, v.ptr), "track_update_poi")
+
+    def track_current_poi(self, poi: DeviceArray, n: int, u: DeviceArray, v: DeviceArray, out: DeviceArray):
+        self._check(lib().twdic_track_current_poi(self.h, poi.ptr, n, u.ptr, v.ptr, out.ptr), "track_current_poi")
+
+    def track_accumulate_uv(self, n: int, u: DeviceArray, v: DeviceArray, acc_u: DeviceArray, acc_v: DeviceArray):
+        self._check(lib().twdic_track_accumulate_uv(self.h, n, u.ptr, v.ptr, acc_u.ptr, acc_v.ptr),
+                    "track_accumulate_uv")
 
     def record(self, slot: int):
         self._check(lib().twdic_event_record(self.h, slot), "event_record")

@@ -1,10 +1,13 @@
 // frame_loop.cpp -- the per-frame driver of TW_Core's real-time variant
 // (FFTCCTWorkerThread ctor + processFrameFFTCC_ICGN,
 // TW/TW_Core/fftcctworkerthread.cpp:9-103, 399-572) written against the
-// drop-in classes: the same constructor calls, reference reset cadence, and
-// the g_cuHandle members handed from cuFFTCC2D to cuICGN2D.  Reads raw 8-bit
-// frames, writes per-frame results (u, v, zncc, iterations, flags, p6) as a
-// binary stream.  Usage:
+// drop-in classes: the same constructor calls, reference reset cadence, the
+// g_cuHandle members handed from cuFFTCC2D to cuICGN2D, and the POI tracking
+// around a reset (accumulated U/V, POIs advanced by the last integer shift).
+// On a reset the current frame becomes the reference (TW_Core dequeues it
+// from its reference buffer).  Reads raw 8-bit frames, writes per frame:
+// u, v, zncc (float), iterations (int32), flags (uint8), p6 (float x 6),
+// current POIs (int32 {y, x}), accumulated u, v (float).  Usage:
 //   tw_frame_loop W H roiX roiY roiW roiH S grid margin N refPeriod out.bin ref.raw tar1.raw [tar2.raw ...]
 #include <cstdio>
 #include <cstdlib>
@@ -12,6 +15,7 @@
 #include <vector>
 
 #include "tw/TW_paDIC_cuFFTCC2D.h"
+#include "tw/TW_cuda_utils.h"
 #include "tw/TW_paDIC_cuICGN2D.h"
 #include "twdic.h"
 
@@ -52,10 +56,22 @@ int main(int argc, char** argv)
         fftcc.cuInitializeFFTCC(d_U, d_V, d_ZNCC, refMat);             // :89
         icgn.cuInitialize(fftcc.g_cuHandle.m_d_fRefImg);
         const int n = fftcc.GetNumPOIs();
+        // fftcctworkerthread.cpp:79-90: accumulators zeroed, current POIs = POIs
+        twdic_ctx* fc = fftcc.context();
+        real_t *d_accU = nullptr, *d_accV = nullptr;
+        int* d_curPOI = nullptr;
+        twdic_malloc(fc, (void**)&d_accU, n * sizeof(real_t));
+        twdic_malloc(fc, (void**)&d_accV, n * sizeof(real_t));
+        twdic_malloc(fc, (void**)&d_curPOI, 2 * n * sizeof(int));
+        twdic_memset(fc, d_accU, 0, n * sizeof(real_t));
+        twdic_memset(fc, d_accV, 0, n * sizeof(real_t));
+        twdic_memcpy(fc, d_curPOI, fftcc.g_cuHandle.m_d_iPOIXY, 2 * n * sizeof(int), 2);
+        twdic_sync(fc);
         FILE* fo = std::fopen(out, "wb");
         if (!fo) throw std::runtime_error("cannot open output");
         std::vector<float> u(n), v(n), z(n), p6(6 * n);
-        std::vector<int> it(n);
+        std::vector<int> it(n), cur(2 * n);
+        std::vector<float> au(n), av(n);
         std::vector<unsigned char> fl(n);
         for (int k = 13 + 1, count = 1; k < argc; ++k, ++count) {
             auto tar = read_raw(argv[k], (size_t)W * H);
@@ -63,24 +79,38 @@ int main(int argc, char** argv)
             if (count % period == 1 && count > 1) {                    // :408-443
                 fftcc.ResetRefImg(tarMat);
                 icgn.cuInitialize(fftcc.g_cuHandle.m_d_fRefImg);
+                // :419-437 (count > period here): accumulate the last frame's
+                // U/V, then advance the POIs by its integer part
+                cuAccumulateUV(d_accU, d_accV, n, d_U, d_V);
+                cuUpdatePOIpos(d_U, d_V, fftcc.GetNumPOIsX(), fftcc.GetNumPOIsY(), fftcc.g_cuHandle.m_d_iPOIXY);
             }
             fftcc.cuComputeFFTCC(d_U, d_V, d_ZNCC, tarMat);           // :449-453
             twdic_memcpy(fftcc.context(), z.data(), d_ZNCC, n * 4, 1);
             icgn.cuCompute(fftcc.g_cuHandle.m_d_fTarImg, fftcc.g_cuHandle.m_d_iPOIXY, d_U, d_V);   // :457-461
+            cuAccumulatePOI(d_U, d_V, fftcc.g_cuHandle.m_d_iPOIXY, n, d_curPOI);   // :476-481
             twdic_ctx* c = icgn.context();
             twdic_memcpy(c, u.data(), d_U, n * 4, 1);
             twdic_memcpy(c, v.data(), d_V, n * 4, 1);
             twdic_memcpy(c, it.data(), icgn.g_cuHandleICGN.m_d_iIterationNums, n * 4, 1);
             twdic_memcpy(c, fl.data(), icgn.g_cuHandleICGN.m_d_iFlags, n, 1);
             twdic_memcpy(c, p6.data(), icgn.g_cuHandleICGN.m_d_dP, n * 24, 1);
+            twdic_memcpy(c, cur.data(), d_curPOI, n * 8, 1);
+            twdic_memcpy(c, au.data(), d_accU, n * 4, 1);
+            twdic_memcpy(c, av.data(), d_accV, n * 4, 1);
             std::fwrite(u.data(), 4, n, fo);
             std::fwrite(v.data(), 4, n, fo);
             std::fwrite(z.data(), 4, n, fo);
             std::fwrite(it.data(), 4, n, fo);
             std::fwrite(fl.data(), 1, n, fo);
             std::fwrite(p6.data(), 4, 6 * n, fo);
+            std::fwrite(cur.data(), 4, 2 * n, fo);
+            std::fwrite(au.data(), 4, n, fo);
+            std::fwrite(av.data(), 4, n, fo);
         }
         std::fclose(fo);
+        twdic_free(fc, d_accU);
+        twdic_free(fc, d_accV);
+        twdic_free(fc, d_curPOI);
         fftcc.cuDestroyFFTCC(d_U, d_V, d_ZNCC);
         std::printf("frame_loop ok: %d POIs, %d frames\n", n, argc - 14);
     } catch (const std::exception& e) {

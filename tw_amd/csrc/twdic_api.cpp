@@ -368,6 +368,41 @@ twdic_status twdic_get_buffer(twdic_ctx* c, int which, void** d_ptr, size_t* byt
     return TWDIC_OK;
 }
 
+// ctx == NULL: default stream, synchronous (the reference's wrappers use the
+// legacy default stream).
+static twdic_status track_finish(twdic_ctx* c, hipError_t e)
+{
+    if (e == hipSuccess && !c) e = hipStreamSynchronize(nullptr);
+    if (e != hipSuccess) {
+        if (c) return hip_fail(c, e, "tracking kernel");
+        return TWDIC_EHIP;
+    }
+    return TWDIC_OK;
+}
+
+twdic_status twdic_track_update_poi(twdic_ctx* c, int32_t* d_poi, int n, const float* d_u, const float* d_v)
+{
+    if (n < 0 || (n > 0 && (!d_poi || !d_u || !d_v))) return c ? fail(c, TWDIC_EINVAL, "track_update_poi: bad arguments") : TWDIC_EINVAL;
+    if (c && c->refstate_poi == d_poi) c->refstate_version = ~0ull;     // the POIs moved
+    return track_finish(c, launch_track_update_poi(d_poi, n, d_u, d_v, c ? c->stream : nullptr));
+}
+
+twdic_status twdic_track_current_poi(twdic_ctx* c, const int32_t* d_poi, int n, const float* d_u, const float* d_v,
+                                     int32_t* d_out)
+{
+    if (n < 0 || (n > 0 && (!d_poi || !d_u || !d_v || !d_out)))
+        return c ? fail(c, TWDIC_EINVAL, "track_current_poi: bad arguments") : TWDIC_EINVAL;
+    return track_finish(c, launch_track_current_poi(d_poi, n, d_u, d_v, d_out, c ? c->stream : nullptr));
+}
+
+twdic_status twdic_track_accumulate_uv(twdic_ctx* c, int n, const float* d_u, const float* d_v, float* d_acc_u,
+                                       float* d_acc_v)
+{
+    if (n < 0 || (n > 0 && (!d_u || !d_v || !d_acc_u || !d_acc_v)))
+        return c ? fail(c, TWDIC_EINVAL, "track_accumulate_uv: bad arguments") : TWDIC_EINVAL;
+    return track_finish(c, launch_track_accumulate_uv(n, d_u, d_v, d_acc_u, d_acc_v, c ? c->stream : nullptr));
+}
+
 twdic_status twdic_malloc(twdic_ctx* c, void** d_ptr, size_t bytes)
 {
     if (!c || !d_ptr) return TWDIC_EINVAL;
