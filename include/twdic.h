@@ -190,6 +190,30 @@ twdic_status twdic_track_current_poi(twdic_ctx* ctx, const int32_t* d_poi_yx, in
 twdic_status twdic_track_accumulate_uv(twdic_ctx* ctx, int n, const float* d_u, const float* d_v, float* d_acc_u,
                                        float* d_acc_v);
 
+/* --------------------------------------------------------------- results */
+
+/* Writes the per-POI result table in the 14-column CSV layout of the
+ * reference's result files (TW/MATLAB/Sample 14 L1 Amp0_8.3_Data.csv, read by
+ * calcustrain4_x_0_L1_5_40.m:5-6): x, y, integer u, u, du/dx, du/dy,
+ * integer v, v, dv/dx, dv/dy, iterations, ZNCC, FFT-CC status, IC-GN flag;
+ * one row per POI in POI order (x fastest), reals with 6 significant digits.
+ * All pointers are HOST arrays of n entries (p6: n x 6 = u, ux, uy, v, vx,
+ * vy; poi_yx: n x 2 = y, x).  int_u/int_v are the FFT-CC shifts.  zncc,
+ * fftcc_status and flags may be NULL (written as 0). */
+twdic_status twdic_write_csv(const char* path, int n, const int32_t* poi_yx, const float* int_u,
+                             const float* int_v, const float* p6, const int32_t* iters, const float* zncc,
+                             const uint8_t* fftcc_status, const uint8_t* flags);
+
+/* Least-squares strain of a displacement field on the POI grid, as
+ * calcustrain4_x_0_L1_5_40.m:23-33 computes it: for every grid point with a
+ * full (2M+1) x (2M+1) window, d/dx = 3 / (G M (M+1) (2M+1)^2) sum ii f and
+ * d/dy the same with jj (grid offsets ii along x, jj along y; G = grid step
+ * in pixels), in FP64.  d_field: device float[ny][nx]; outputs device
+ * double[ny-2M][nx-2M] (either may be NULL), per pixel (the script reports
+ * x 1e6, microstrain).  Asynchronous on ctx's stream. */
+twdic_status twdic_strain_ls(twdic_ctx* ctx, const float* d_field, int nx, int ny, int grid, int half_window,
+                             double* d_dfdx, double* d_dfdy);
+
 /* ------------------------------------------------ precompute (inspection) */
 
 /* Device buffers the reference exposes as public members of

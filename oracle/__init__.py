@@ -56,6 +56,7 @@ def lib() -> C.CDLL:
             "orc_track_update_poi": ([vp, i, vp, vp], None),
             "orc_track_current_poi": ([vp, i, vp, vp, vp], None),
             "orc_track_accumulate_uv": ([i, vp, vp, vp, vp], None),
+            "orc_strain_ls": ([vp, i, i, i, i, vp, vp], None),
         }
         for name, (a, r) in sig.items():
             fn = getattr(L, name)
@@ -219,3 +220,14 @@ def track_accumulate_uv(acc_u, acc_v, u, v):
     v = np.ascontiguousarray(v, np.float32)
     lib().orc_track_accumulate_uv(len(au), _p(u), _p(v), _p(au), _p(av))
     return au, av
+
+
+def strain_ls(field, grid, M):
+    """(d/dx, d/dy) of field[ny][nx] by the least-squares window of
+    calcustrain4_x_0_L1_5_40.m, per pixel."""
+    f = np.ascontiguousarray(field, np.float32)
+    ny, nx = f.shape
+    gx = np.empty((ny - 2 * M, nx - 2 * M), np.float64)
+    gy = np.empty_like(gx)
+    lib().orc_strain_ls(_p(f), nx, ny, grid, M, _p(gx), _p(gy))
+    return gx, gy

@@ -166,3 +166,25 @@ void orc_track_accumulate_uv(int n, const float* u, const float* v, float* acc_u
         acc_v[i] += v[i];
     }
 }
+
+/* F3: TW/MATLAB/calcustrain4_x_0_L1_5_40.m:23-33. */
+void orc_strain_ls(const float* f, int nx, int ny, int grid, int M, double* dfdx, double* dfdy)
+{
+    const int ox = nx - 2 * M, oy = ny - 2 * M;
+    const double w = (double)(2 * M + 1);
+    const double k = 3.0 / ((double)grid * (double)M * (double)(M + 1) * (w * w));
+    for (int j = M; j < ny - M; ++j)
+        for (int i = M; i < nx - M; ++i) {
+            double sx = 0.0, sy = 0.0;
+            for (int ii = -M; ii <= M; ++ii)
+                for (int jj = -M; jj <= M; ++jj) {
+                    double v = (double)f[(size_t)(j + jj) * nx + (i + ii)];
+                    double a = (double)ii * v, b = (double)jj * v;
+                    sx = sx + a;
+                    sy = sy + b;
+                }
+            dfdx[(size_t)(j - M) * ox + (i - M)] = k * sx;
+            dfdy[(size_t)(j - M) * ox + (i - M)] = k * sy;
+        }
+    (void)oy;
+}

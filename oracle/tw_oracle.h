@@ -150,6 +150,12 @@ void orc_track_update_poi(int32_t* poi_yx, int n, const float* u, const float* v
 void orc_track_current_poi(const int32_t* poi_yx, int n, const float* u, const float* v, int32_t* out_yx);
 void orc_track_accumulate_uv(int n, const float* u, const float* v, float* acc_u, float* acc_v);
 
+/* F3: least-squares strain on the POI grid, calcustrain4_x_0_L1_5_40.m:23-33
+ * (MATLAB, double): for each point with a full (2M+1)^2 window,
+ * d/dx = 3/(G M (M+1) (2M+1)^2) * sum_{ii} sum_{jj} ii * f(i+ii, j+jj), ii outer;
+ * d/dy the same weighted by jj.  f is [ny][nx]; outputs [ny-2M][nx-2M]. */
+void orc_strain_ls(const float* f, int nx, int ny, int grid, int M, double* dfdx, double* dfdy);
+
 #ifdef __cplusplus
 }
 #endif

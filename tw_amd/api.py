@@ -88,6 +88,8 @@ _SIGS = {
     "twdic_track_update_poi": ([_VP, _VP, _I, _VP, _VP], _I),
     "twdic_track_current_poi": ([_VP, _VP, _I, _VP, _VP, _VP], _I),
     "twdic_track_accumulate_uv": ([_VP, _I, _VP, _VP, _VP, _VP], _I),
+    "twdic_write_csv": ([C.c_char_p, _I, _VP, _VP, _VP, _VP, _VP, _VP, _VP, _VP], _I),
+    "twdic_strain_ls": ([_VP, _VP, _I, _I, _I, _I, _VP, _VP], _I),
     "twdic_event_record": ([_VP, _I], _I),
     "twdic_event_elapsed": ([_VP, _I, _I, C.POINTER(C.c_float)], _I),
     "twdic_batch_create": ([_I, C.POINTER(_VP)], _I),
@@ -313,6 +315,11 @@ class Context:
         self._check(lib().twdic_track_accumulate_uv(self.h, n, u.ptr, v.ptr, acc_u.ptr, acc_v.ptr),
                     "track_accumulate_uv")
 
+    def strain_ls(self, field: DeviceArray, nx: int, ny: int, grid: int, M: int, dfdx: DeviceArray | None,
+                  dfdy: DeviceArray | None):
+        g = lambda a: a.ptr if a is not None else None  # noqa: E731
+        self._check(lib().twdic_strain_ls(self.h, field.ptr, nx, ny, grid, M, g(dfdx), g(dfdy)), "strain_ls")
+
     def record(self, slot: int):
         self._check(lib().twdic_event_record(self.h, slot), "event_record")
 
@@ -332,6 +339,19 @@ def num_poi(cfg: Config) -> tuple[int, int]:
     nx, ny = C.c_int32(), C.c_int32()
     lib().twdic_num_poi(C.byref(cfg), C.byref(nx), C.byref(ny))
     return nx.value, ny.value
+
+
+def write_csv(path, poi_yx, int_u, int_v, p6, iters, zncc=None, fftcc_status=None, flags=None):
+    """Per-POI result table in the reference's 14-column CSV layout."""
+    a = lambda x, dt: None if x is None else np.ascontiguousarray(x, dt)  # noqa: E731
+    poi, iu, iv = a(poi_yx, np.int32), a(int_u, np.float32), a(int_v, np.float32)
+    p6, it, z = a(p6, np.float32), a(iters, np.int32), a(zncc, np.float32)
+    st, fl = a(fftcc_status, np.uint8), a(flags, np.uint8)
+    n = len(poi)
+    g = lambda x: None if x is None else _ptr(x)  # noqa: E731
+    r = lib().twdic_write_csv(str(path).encode(), n, g(poi), g(iu), g(iv), g(p6), g(it), g(z), g(st), g(fl))
+    if r != OK:
+        raise TwdicError(f"twdic_write_csv failed with status {r}")
 
 
 class Batch:

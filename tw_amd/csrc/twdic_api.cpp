@@ -403,6 +403,32 @@ twdic_status twdic_track_accumulate_uv(twdic_ctx* c, int n, const float* d_u, co
     return track_finish(c, launch_track_accumulate_uv(n, d_u, d_v, d_acc_u, d_acc_v, c ? c->stream : nullptr));
 }
 
+twdic_status twdic_write_csv(const char* path, int n, const int32_t* poi, const float* iu, const float* iv,
+                             const float* p6, const int32_t* iters, const float* zncc, const uint8_t* fst,
+                             const uint8_t* flags)
+{
+    if (!path || n < 0 || (n > 0 && (!poi || !iu || !iv || !p6 || !iters))) return TWDIC_EINVAL;
+    FILE* f = std::fopen(path, "w");
+    if (!f) return TWDIC_EINVAL;
+    for (int i = 0; i < n; ++i) {
+        const float* p = p6 + 6 * (size_t)i;
+        std::fprintf(f, "%d,%d,%d,%g,%g,%g,%d,%g,%g,%g,%d,%g,%d,%d\n", poi[2 * i + 1], poi[2 * i], (int)iu[i],
+                     (double)p[0], (double)p[1], (double)p[2], (int)iv[i], (double)p[3], (double)p[4], (double)p[5],
+                     iters[i], zncc ? (double)zncc[i] : 0.0, fst ? (int)fst[i] : 0, flags ? (int)flags[i] : 0);
+    }
+    const bool ok = std::ferror(f) == 0;
+    return (std::fclose(f) == 0 && ok) ? TWDIC_OK : TWDIC_EINVAL;
+}
+
+twdic_status twdic_strain_ls(twdic_ctx* c, const float* d_field, int nx, int ny, int grid, int M, double* d_dfdx,
+                             double* d_dfdy)
+{
+    if (!c || !d_field || nx < 1 || ny < 1 || grid < 1 || M < 1 || 2 * M + 1 > nx || 2 * M + 1 > ny)
+        return c ? fail(c, TWDIC_EINVAL, "strain_ls: bad arguments") : TWDIC_EINVAL;
+    TW_HIP(c, launch_strain_ls(d_field, nx, ny, grid, M, d_dfdx, d_dfdy, c->stream));
+    return TWDIC_OK;
+}
+
 twdic_status twdic_malloc(twdic_ctx* c, void** d_ptr, size_t bytes)
 {
     if (!c || !d_ptr) return TWDIC_EINVAL;

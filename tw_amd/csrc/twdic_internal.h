@@ -119,6 +119,8 @@ hipError_t launch_batch_ref_state(const PairDesc* pairs, const int2* items, int 
 hipError_t launch_batch_fftcc(const PairDesc* pairs, const int2* items, int total, int* counter, int N,
                               int variant, int blocks, hipStream_t st);
 hipError_t launch_fill_zero(int* p, int n, hipStream_t st);
+hipError_t launch_strain_ls(const float* f, int nx, int ny, int grid, int M, double* dfdx, double* dfdy,
+                            hipStream_t st);
 // POI tracking across reference resets
 hipError_t launch_track_update_poi(int32_t* poi, int n, const float* u, const float* v, hipStream_t st);
 hipError_t launch_track_current_poi(const int32_t* poi, int n, const float* u, const float* v, int32_t* out,
