@@ -57,6 +57,22 @@ def speckle_2048(seed=1002) -> Case:
     return Case("speckle2048_S20_g5", 2048, 20, 5, 64, (1, 1, 2046, 2046), 13, ref, tar, d)
 
 
+def precision_4096(seed=1004) -> tuple[Case, np.ndarray]:
+    """4096x4096, 0.0153 spots/px^2, sinusoidal field u = 0.5 sin(2 pi x/256),
+    v = 0.3 cos(2 pi y/192), noise sigma 3, spots removed from 10 random
+    200x200 boxes (low texture); 41x41 subsets on a 4 px grid from a 32 px
+    border (1009 x 1009 = 1,018,081 POIs), FFT-CC window 64.  Returns the case
+    and the boxes (int32[10][4] = x0, y0, x1, y1)."""
+    rng = np.random.default_rng(seed)
+    x0 = rng.integers(100, 4096 - 300, 10)
+    y0 = rng.integers(100, 4096 - 300, 10)
+    boxes = np.stack([x0, y0, x0 + 200, y0 + 200], 1).astype(np.int32)
+    d = api.sinusoid(0.5, 256.0, 0.3, 192.0)
+    ref, tar = api.synth_pair(4096, 4096, int(0.0153 * 4096 * 4096), seed, d, noise_sigma=3.0, boxes=boxes)
+    # POI = 1 + 11 + 20 = 32; (4094 - 40 - 22) / 4 + 1 = 1009 per axis
+    return Case("precision4096_S20_g4", 4096, 20, 4, 64, (1, 1, 4094, 4094), 11, ref, tar, d), boxes
+
+
 def truth(case: Case, poi: np.ndarray) -> np.ndarray:
     """Analytic (u, ux, uy, v, vx, vy) at each POI of an affine case."""
     out = np.zeros((len(poi), 6))
