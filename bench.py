@@ -61,6 +61,90 @@ def copy_calibration(mib: int, reps: int):
     print(json.dumps({"copy_bytes": nbytes, "reps": reps}))
 
 
+def serving(a):
+    """Poisson stream of correlation requests through the serving loop: 1024^2
+    frames deformed from a resident reference (strains within +-0.5 %,
+    translations within +-4 px), POI sets of 500..20,000 POIs with subset
+    radius 10, 15 or 20 (N = 32, 32, 64), the reference replaced every 200
+    requests.  Frames and POI sets come from pre-generated pools (8 targets
+    per reference, 4 references, 16 POI sets)."""
+    from tw_amd import api
+    W = H = 1024
+    rng = np.random.default_rng(1005)
+    t_gen = time.perf_counter()
+    refs, tars = [], []
+    for r in range(4):
+        seed = 1005900000 + r
+        ref = None
+        pool = []
+        for k in range(8):
+            d = api.affine(*rng.uniform(-4, 4, 2), *rng.uniform(-0.005, 0.005, 3), W / 2, H / 2)
+            rr, t = api.synth_pair(W, H, int(0.0153 * W * H), seed, d, noise_sigma=2.0,
+                                   noise_seed=1005000000 + 8 * r + k, want_ref=ref is None)
+            ref = rr if ref is None else ref
+            pool.append(t)
+        refs.append(ref)
+        tars.append(pool)
+    sets = []
+    for k in range(16):
+        n = int(rng.integers(500, 20001))
+        sets.append(np.stack([rng.integers(40, H - 40, n), rng.integers(40, W - 40, n)], 1).astype(np.int32))
+    t_gen = time.perf_counter() - t_gen
+    classes = [(10, 32), (15, 32), (20, 64)]
+    with api.Server(slots=a.slots, queue_capacity=1024, max_width=W, max_height=H, max_pois=20000) as srv:
+        ids = [srv.add_poiset(p) for p in sets]
+        outs = [[api.Server.outputs(len(p)) for _ in range(4)] for p in sets]
+        cur = 0
+        srv.set_reference(0, refs[cur])
+        # warm-up: one request of every class and set
+        for i in range(len(sets)):
+            for S, N in classes:
+                srv.submit(0, ids[i], tars[cur][0], S, N, out=outs[i][0])
+        srv.drain()
+        srv.stats_reset()
+        n_req = int(a.rate * a.duration)
+        gaps = rng.exponential(1.0 / a.rate, n_req)
+        t_start = time.perf_counter()
+        t_next = t_start
+        submitted = 0
+        for k in range(n_req):
+            t_next += gaps[k]
+            while True:
+                now = time.perf_counter()
+                if now >= t_next:
+                    break
+                if t_next - now > 2e-4:
+                    time.sleep(t_next - now - 1e-4)
+            if k and k % 200 == 0:                     # new reference every 200 requests
+                cur = (cur + 1) % len(refs)
+                srv.set_reference(0, refs[cur])
+            i = int(rng.integers(0, len(sets)))
+            S, N = classes[int(rng.integers(0, 3))]
+            t, _ = srv.submit(0, ids[i], tars[cur][int(rng.integers(0, 8))], S, N, drop_if_full=True,
+                              out=outs[i][k % 4])
+            submitted += t is not None
+        t_arrivals = time.perf_counter() - t_start
+        srv.drain()
+        t_all = time.perf_counter() - t_start
+        st = srv.stats()
+    out = {
+        "metric": "serving: IC-GN POIs/s and p50/p99 request latency under a Poisson request stream",
+        "value": st.pois / t_all, "unit": "POIs/s", "n_gpus": 1, "higher_is_better": True,
+        "dtype": "fp32", "data": "synthetic",
+        "config": {"workload": "serving_1024_poisson", "rate_req_s": a.rate, "duration_s": a.duration,
+                   "slots": a.slots, "frame": "1024x1024 uint8 (host memory)",
+                   "pois_per_request": "500..20000 (16-set pool)",
+                   "subset_radius": [10, 15, 20], "reference_every": 200, "queue_capacity": 1024},
+        "offered_requests": n_req, "arrival_s": t_arrivals, "wall_s": t_all,
+        "completed": int(st.completed), "dropped": int(st.dropped), "failed": int(st.failed),
+        "requests_per_s": st.completed / t_all,
+        "latency_ms": {"p50": st.lat_p50_ms, "p99": st.lat_p99_ms, "max": st.lat_max_ms},
+        "slots_busy_peak": st.slots_busy_peak, "hbm_peak_bytes": int(st.hbm_peak),
+        "state_builds": int(st.state_builds), "state_hits": int(st.state_hits), "synth_s": t_gen,
+    }
+    print(json.dumps(out))
+
+
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -70,6 +154,12 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sample", type=int, default=0,
                     help="POIs in the CPU baseline sample (0: the whole frame, about 15 s on 16 cores)")
+    ap.add_argument("--workload", choices=["frame", "serving"], default="frame",
+                    help="frame: the 2048^2 plain frame step (default); serving: Poisson request stream "
+                         "through the serving loop")
+    ap.add_argument("--rate", type=float, default=2000.0, help="serving: mean arrivals per second")
+    ap.add_argument("--duration", type=float, default=10.0, help="serving: seconds of arrivals")
+    ap.add_argument("--slots", type=int, default=8, help="serving: requests in flight at once")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a MIB-sized device copy (PMC counter calibration) and exit")
     return ap.parse_args()
@@ -107,6 +197,12 @@ def main():
     a = parse()
     if a.copy_calibration:
         copy_calibration(a.copy_calibration, 4)
+        return
+    if a.workload == "serving":
+        if a.gpus != 1:
+            print("bench.py runs on one GPU only", file=sys.stderr)
+            sys.exit(2)
+        serving(a)
         return
     if a.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {a.gpus}); the multi-GPU layout is a design "

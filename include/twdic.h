@@ -238,6 +238,10 @@ twdic_status twdic_free(twdic_ctx* ctx, void* d_ptr);
  * are asynchronous on the context stream. */
 twdic_status twdic_memcpy(twdic_ctx* ctx, void* dst, const void* src, size_t bytes, int kind);
 twdic_status twdic_memset(twdic_ctx* ctx, void* d_ptr, int value, size_t bytes);
+/* Pinned (page-locked) host memory, for frames and results that move to and
+ * from the device by DMA without a staging copy. */
+twdic_status twdic_host_alloc(void** h_ptr, size_t bytes);
+twdic_status twdic_host_free(void* h_ptr);
 
 /* ------------------------------------------------------------- timing */
 

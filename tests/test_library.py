@@ -22,7 +22,7 @@ def _declared(header: Path):
                                  text, re.M)))
 
 
-@pytest.mark.parametrize("header", ["twdic.h", "twdic_synth.h", "twdic_batch.h"])
+@pytest.mark.parametrize("header", ["twdic.h", "twdic_synth.h", "twdic_batch.h", "twdic_serve.h"])
 def test_library_exports_every_declared_symbol(header, lib_built):
     names = _declared(ROOT / "include" / header)
     assert len(names) >= 2
@@ -36,7 +36,7 @@ def test_library_exports_every_declared_symbol(header, lib_built):
 
 def test_ctypes_signatures_cover_the_abi():
     names = set()
-    for h in ("twdic.h", "twdic_synth.h", "twdic_batch.h"):
+    for h in ("twdic.h", "twdic_synth.h", "twdic_batch.h", "twdic_serve.h"):
         names |= set(_declared(ROOT / "include" / h))
     assert names == set(api.exported_symbols())
 

@@ -57,6 +57,29 @@ class Tuning(C.Structure):
     _fields_ = [("icgn_chunk", C.c_int), ("fftcc_variant", C.c_int), ("use_graph", C.c_int)]
 
 
+class ServerConfig(C.Structure):
+    _fields_ = [("device", C.c_int), ("slots", C.c_int), ("queue_capacity", C.c_int), ("hbm_budget", C.c_size_t),
+                ("max_width", C.c_int), ("max_height", C.c_int), ("max_pois", C.c_int)]
+
+
+class Request(C.Structure):
+    _fields_ = [("ref_id", C.c_int), ("poiset", C.c_int), ("target", C.c_void_p), ("stride", C.c_int),
+                ("subset", C.c_int), ("fft_n", C.c_int), ("max_iter", C.c_int), ("delta_p", C.c_float),
+                ("u", C.c_void_p), ("v", C.c_void_p), ("zncc", C.c_void_p), ("p6", C.c_void_p),
+                ("iters", C.c_void_p), ("flags", C.c_void_p), ("memory", C.c_int)]
+
+
+MEM_PAGEABLE, MEM_PINNED, MEM_DEVICE = 0, 1, 2
+
+
+class ServerStats(C.Structure):
+    _fields_ = [("submitted", C.c_uint64), ("admitted", C.c_uint64), ("completed", C.c_uint64),
+                ("dropped", C.c_uint64), ("failed", C.c_uint64), ("pois", C.c_double), ("lat_p50_ms", C.c_double),
+                ("lat_p99_ms", C.c_double), ("lat_max_ms", C.c_double), ("wall_s", C.c_double),
+                ("hbm_budget", C.c_size_t), ("hbm_in_use", C.c_size_t), ("hbm_peak", C.c_size_t),
+                ("slots_busy_peak", C.c_int), ("state_builds", C.c_uint64), ("state_hits", C.c_uint64)]
+
+
 class BatchInfo(C.Structure):
     _fields_ = [("npairs", C.c_int), ("total_pois", C.c_int64), ("fftcc_items", C.c_int64 * 3),
                 ("device_bytes", C.c_size_t), ("tuning", Tuning)]
@@ -85,6 +108,8 @@ _SIGS = {
     "twdic_free": ([_VP, _VP], _I),
     "twdic_memcpy": ([_VP, _VP, _VP, _SZ, _I], _I),
     "twdic_memset": ([_VP, _VP, _I, _SZ], _I),
+    "twdic_host_alloc": ([C.POINTER(_VP), _SZ], _I),
+    "twdic_host_free": ([_VP], _I),
     "twdic_track_update_poi": ([_VP, _VP, _I, _VP, _VP], _I),
     "twdic_track_current_poi": ([_VP, _VP, _I, _VP, _VP, _VP], _I),
     "twdic_track_accumulate_uv": ([_VP, _I, _VP, _VP, _VP, _VP], _I),
@@ -109,6 +134,17 @@ _SIGS = {
     "twdic_batch_info_get": ([_VP, C.POINTER(BatchInfo)], _I),
     "twdic_batch_record": ([_VP, _I], _I),
     "twdic_batch_elapsed": ([_VP, _I, _I, C.POINTER(C.c_float)], _I),
+    "twdic_server_create": ([C.POINTER(ServerConfig), C.POINTER(_VP)], _I),
+    "twdic_server_destroy": ([_VP], _I),
+    "twdic_server_last_error": ([_VP], C.c_char_p),
+    "twdic_server_set_reference": ([_VP, _I, _VP, _I, _I, _I], _I),
+    "twdic_server_add_poiset": ([_VP, _VP, _I, C.POINTER(_I)], _I),
+    "twdic_server_submit": ([_VP, C.POINTER(Request), _I, C.POINTER(C.c_uint64)], _I),
+    "twdic_server_poll": ([_VP, C.c_uint64, C.POINTER(_I)], _I),
+    "twdic_server_wait": ([_VP, C.c_uint64], _I),
+    "twdic_server_drain": ([_VP], _I),
+    "twdic_server_stats_get": ([_VP, C.POINTER(ServerStats)], _I),
+    "twdic_server_stats_reset": ([_VP], _I),
     "twdic_synth_pair": ([C.POINTER(Speckle), C.POINTER(Deform), _VP, _VP, _I], _I),
     "twdic_synth_displacement": ([C.POINTER(Deform), C.c_double, C.c_double, C.POINTER(C.c_double)], None),
 }
@@ -434,3 +470,133 @@ class Batch:
         ms = C.c_float()
         self._check(lib().twdic_batch_elapsed(self.h, a, b, C.byref(ms)), "elapsed")
         return ms.value
+
+
+class _Pinned:
+    def __init__(self, nbytes):
+        p = C.c_void_p()
+        if lib().twdic_host_alloc(C.byref(p), max(1, nbytes)) != OK:
+            raise TwdicError("twdic_host_alloc failed")
+        self.ptr = p.value
+
+    def __del__(self):
+        if self.ptr:
+            lib().twdic_host_free(self.ptr)
+            self.ptr = None
+
+
+class PinnedArray(np.ndarray):
+    """ndarray over pinned host memory; the allocation lives as long as the
+    array (views keep their base alive)."""
+
+
+def pinned_empty(shape, dtype) -> np.ndarray:
+    """A numpy array in pinned host memory (freed with the array)."""
+    dtype = np.dtype(dtype)
+    shape = tuple(np.atleast_1d(shape).tolist()) if not isinstance(shape, tuple) else shape
+    nbytes = int(np.prod(shape)) * dtype.itemsize
+    buf = _Pinned(nbytes)
+    raw = (C.c_char * max(1, nbytes)).from_address(buf.ptr)
+    a = np.frombuffer(raw, dtype=np.uint8, count=nbytes).view(dtype).reshape(shape).view(PinnedArray)
+    a._owner = buf
+    return a
+
+
+class Server:
+    """A twdic_server: continuous-batching serving loop over resident
+    references and POI sets.  Host arrays handed to submit() are kept alive
+    until the request has been waited for."""
+
+    def __init__(self, slots=8, queue_capacity=256, hbm_budget=0, max_width=2048, max_height=2048,
+                 max_pois=20000, device=0):
+        cfg = ServerConfig(device, slots, queue_capacity, hbm_budget, max_width, max_height, max_pois)
+        h = C.c_void_p()
+        st = lib().twdic_server_create(C.byref(cfg), C.byref(h))
+        if st != OK:
+            raise TwdicError(f"twdic_server_create failed with status {st}")
+        self.h = h.value
+        self._live = {}
+        self._poi_n = []
+
+    def _check(self, st, what=""):
+        if st != OK:
+            raise TwdicError(f"{what}: status {st}: {lib().twdic_server_last_error(self.h).decode()}")
+
+    def close(self):
+        if self.h:
+            lib().twdic_server_destroy(self.h)
+            self.h = None
+            self._live.clear()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def set_reference(self, ref_id: int, img: np.ndarray):
+        img = np.ascontiguousarray(img, np.uint8)
+        self._check(lib().twdic_server_set_reference(self.h, ref_id, _ptr(img), img.shape[1], img.shape[0],
+                                                     img.shape[1]), "set_reference")
+
+    def add_poiset(self, poi_yx: np.ndarray) -> int:
+        poi = np.ascontiguousarray(poi_yx, np.int32)
+        i = C.c_int()
+        self._check(lib().twdic_server_add_poiset(self.h, _ptr(poi), len(poi), C.byref(i)), "add_poiset")
+        self._poi_n.append(len(poi))
+        return i.value
+
+    @staticmethod
+    def outputs(n: int, pinned: bool = False) -> dict:
+        mk = (lambda shp, dt: pinned_empty(shp, dt)) if pinned else (lambda shp, dt: np.zeros(shp, dt))
+        return dict(u=mk(n, np.float32), v=mk(n, np.float32), zncc=mk(n, np.float32), p6=mk((n, 6), np.float32),
+                    iters=mk(n, np.int32), flags=mk(n, np.uint8))
+
+    def submit(self, ref_id: int, poiset: int, target, subset: int, fft_n: int, max_iter=20,
+               delta_p=0.001, drop_if_full=False, out: dict | None = None, memory: int = MEM_PAGEABLE,
+               stride: int | None = None):
+        """Returns (ticket, outputs dict) or (None, None) when dropped.  `out`
+        (from outputs(n)) is reused when given.  With memory=MEM_DEVICE the
+        target and outputs are DeviceArrays (stride = target row bytes)."""
+        n = self._poi_n[poiset]
+        if out is None:
+            out = self.outputs(n)
+        if isinstance(target, DeviceArray):
+            tp, st_ = target.ptr, stride if stride is not None else target.shape[-1]
+            keep = target
+        else:
+            keep = np.ascontiguousarray(target, np.uint8) if memory == MEM_PAGEABLE else target
+            tp, st_ = _ptr(keep), keep.shape[1]
+        pt = {k: (a.ptr if isinstance(a, DeviceArray) else _ptr(a)) for k, a in out.items()}
+        r = Request(ref_id, poiset, tp, st_, subset, fft_n, max_iter, delta_p, pt["u"], pt["v"], pt["zncc"],
+                    pt["p6"], pt["iters"], pt["flags"], memory)
+        t = C.c_uint64()
+        st = lib().twdic_server_submit(self.h, C.byref(r), 1 if drop_if_full else 0, C.byref(t))
+        if st == EBUSY:
+            return None, None
+        self._check(st, "submit")
+        self._live[t.value] = (keep, out)
+        return t.value, out
+
+    def poll(self, ticket: int) -> int:
+        s = C.c_int()
+        self._check(lib().twdic_server_poll(self.h, ticket, C.byref(s)), "poll")
+        if s.value != 0:
+            self._live.pop(ticket, None)
+        return s.value
+
+    def wait(self, ticket: int):
+        st = lib().twdic_server_wait(self.h, ticket)
+        self._live.pop(ticket, None)
+        self._check(st, "wait")
+
+    def drain(self):
+        self._check(lib().twdic_server_drain(self.h), "drain")
+
+    def stats(self) -> ServerStats:
+        s = ServerStats()
+        self._check(lib().twdic_server_stats_get(self.h, C.byref(s)), "stats")
+        return s
+
+    def stats_reset(self):
+        self._check(lib().twdic_server_stats_reset(self.h), "stats_reset")

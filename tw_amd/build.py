@@ -40,7 +40,7 @@ CXX_FLAGS = ["-O3", "-std=c++17", "-fPIC", "-Wall", "-fopenmp", f"-I{INCLUDE}", 
 
 def _sources():
     hip = sorted(CSRC.glob("*.hip"))
-    hip_host = [CSRC / "twdic_api.cpp", CSRC / "batch_api.cpp"] + sorted(CPPSRC.glob("*.cpp"))
+    hip_host = [CSRC / "twdic_api.cpp", CSRC / "batch_api.cpp", CSRC / "serve.cpp"] + sorted(CPPSRC.glob("*.cpp"))
     host = [CSRC / "synth.cpp"]
     return hip, hip_host, host
 

@@ -458,6 +458,19 @@ twdic_status twdic_memcpy(twdic_ctx* c, void* dst, const void* src, size_t bytes
     return TWDIC_OK;
 }
 
+twdic_status twdic_host_alloc(void** h_ptr, size_t bytes)
+{
+    if (!h_ptr) return TWDIC_EINVAL;
+    *h_ptr = nullptr;
+    return hipHostMalloc(h_ptr, bytes ? bytes : 1, hipHostMallocDefault) == hipSuccess ? TWDIC_OK : TWDIC_ENOMEM;
+}
+
+twdic_status twdic_host_free(void* h_ptr)
+{
+    if (!h_ptr) return TWDIC_OK;
+    return hipHostFree(h_ptr) == hipSuccess ? TWDIC_OK : TWDIC_EHIP;
+}
+
 twdic_status twdic_memset(twdic_ctx* c, void* d_ptr, int value, size_t bytes)
 {
     if (!c || (bytes && !d_ptr)) return TWDIC_EINVAL;
