@@ -11,7 +11,10 @@
 // FFT-CC owner -- this class never frees what it did not allocate).
 // Extensions: ICGN2D_Algorithm mirrors ICGN2D_CPU's host-array interface
 // (TW_paDIC_ICGN2D_CPU.h:56-70) so a CPU IC-GN worker can switch to the GPU;
-// setInterpolationSource selects the LUT gather or the fused evaluation.
+// setInterpolationSource selects the LUT gather or the fused evaluation of
+// the bicubic interpolant; ICGN2DInterpolationFLag::BicubicSpline uses the
+// cubic B-spline LUT of the reference's CPU path (needs a 2-pixel frame right
+// of and below the ROI).
 #ifndef TWAMD_TW_PADIC_CUICGN2D_H
 #define TWAMD_TW_PADIC_CUICGN2D_H
 

@@ -44,6 +44,11 @@ def lib() -> C.CDLL:
             "orc_fftcc_exact": ([vp, vp, i, vp, i, i, vp, vp, vp, vp, i], i),
             "orc_icgn_f32": ([vp, vp, i, i, i, i, i, i, i, i, i, i, f, vp, i, vp, vp, vp, vp, vp, vp, vp, i], i),
             "orc_icgn_f64": ([vp, vp, i, i, i, i, i, i, i, i, i, i, d, vp, i, vp, vp, vp, vp, vp, vp, vp, i], i),
+            "orc_icgn_interp_f32": ([vp, vp, i, i, i, i, i, i, i, i, i, i, f, vp, i, vp, vp, vp, vp, vp, vp, vp, i, i],
+                                    i),
+            "orc_icgn_interp_f64": ([vp, vp, i, i, i, i, i, i, i, i, i, i, d, vp, i, vp, vp, vp, vp, vp, vp, vp, i, i],
+                                    i),
+            "orc_bspline_lut_f32": ([vp, i, i, i, i, i, i, i, vp], i),
             "orc_fft1d_f32": ([vp, i, i], None),
             "orc_fft1d_f64": ([vp, i, i], None),
             "orc_potrf6_f32": ([vp], i),
@@ -139,8 +144,9 @@ def fftcc_exact(ref, tar, poi_yx, N, nthreads=0):
 
 
 def icgn(ref, tar, roi, sx, sy, poi_yx, u0, v0, max_iter=20, delta_p=0.001, precision="f32", nthreads=0,
-         iters_init=-1):
-    """roi = (start_x, start_y, roi_w, roi_h).  Returns dict of per-POI outputs."""
+         iters_init=-1, interp="bicubic"):
+    """roi = (start_x, start_y, roi_w, roi_h); interp "bicubic" or "bspline".
+    Returns dict of per-POI outputs."""
     ref = np.ascontiguousarray(ref, np.uint8)
     tar = np.ascontiguousarray(tar, np.uint8)
     poi = np.ascontiguousarray(poi_yx, np.int32)
@@ -153,10 +159,11 @@ def icgn(ref, tar, roi, sx, sy, poi_yx, u0, v0, max_iter=20, delta_p=0.001, prec
     flags = np.full(n, 255, np.uint8)
     dpn = np.zeros(n, np.float64)
     times = np.zeros(2, np.float64)
-    fn = lib().orc_icgn_f32 if precision == "f32" else lib().orc_icgn_f64
+    fn = lib().orc_icgn_interp_f32 if precision == "f32" else lib().orc_icgn_interp_f64
     h, w = ref.shape
     rc = fn(_p(ref), _p(tar), w, w, h, roi[0], roi[1], roi[2], roi[3], sx, sy, max_iter, delta_p, _p(poi), n,
-            _p(u), _p(v), _p(p6), _p(iters), _p(flags), _p(dpn), _p(times), nthreads)
+            _p(u), _p(v), _p(p6), _p(iters), _p(flags), _p(dpn), _p(times), nthreads,
+            {"bicubic": 0, "bspline": 1}[interp])
     if rc:
         raise ValueError("bad geometry")
     return dict(u=u, v=v, p6=p6, iters=iters, flags=flags, dpnorm=dpn, times_ms=times)
@@ -231,3 +238,13 @@ def strain_ls(field, grid, M):
     gy = np.empty_like(gx)
     lib().orc_strain_ls(_p(f), nx, ny, grid, M, _p(gx), _p(gy))
     return gx, gy
+
+
+def bspline_lut(img, start_x, start_y, roi_w, roi_h) -> np.ndarray:
+    """BicubicSplineCoefficients_s as a [roi_h][roi_w][16] LUT (4k+l ~ y^k x^l)."""
+    img = np.ascontiguousarray(img, np.uint8)
+    h, w = img.shape
+    lut = np.zeros((roi_h, roi_w, 16), np.float32)
+    if lib().orc_bspline_lut_f32(_p(img), w, w, h, start_x, start_y, roi_w, roi_h, _p(lut)):
+        raise ValueError("the ROI needs a 1-pixel frame left/top and 2 pixels right/bottom")
+    return lut

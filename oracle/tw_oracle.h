@@ -105,6 +105,27 @@ int orc_fftcc_exact(const uint8_t* ref, const uint8_t* tar, int stride,
  * last weighted |dp| evaluated by the loop test (NaN if none).
  * times_ms (may be NULL): {precompute, iterations} wall times.
  * Returns 0, or -1 on bad geometry. */
+/* F2: BicubicSplineCoefficients_s (TW_utils.cpp:521-602) into a [roi_h][roi_w]
+ * [16] LUT laid out as the bicubic one.  Needs a 1-pixel frame left/top and
+ * a 2-pixel frame right/bottom; returns -1 otherwise. */
+int orc_bspline_lut_f32(const uint8_t* img, int stride, int img_w, int img_h,
+                        int start_x, int start_y, int roi_w, int roi_h, float* lut);
+/* IC-GN as orc_icgn_* with the target interpolation chosen by interp:
+ * 0 = Bicubic, 1 = BicubicSpline (TW_paDIC_ICGN2D_CPU.cpp:322-341). */
+int orc_icgn_interp_f32(const uint8_t* ref, const uint8_t* tar, int stride,
+                        int img_w, int img_h, int start_x, int start_y,
+                        int roi_w, int roi_h, int sx, int sy, int max_iter,
+                        float delta_p, const int32_t* poi_yx, int n,
+                        float* u, float* v, float* p6, int32_t* iters,
+                        uint8_t* flags, double* dpnorm, double* times_ms,
+                        int nthreads, int interp);
+int orc_icgn_interp_f64(const uint8_t* ref, const uint8_t* tar, int stride,
+                        int img_w, int img_h, int start_x, int start_y,
+                        int roi_w, int roi_h, int sx, int sy, int max_iter,
+                        double delta_p, const int32_t* poi_yx, int n,
+                        double* u, double* v, double* p6, int32_t* iters,
+                        uint8_t* flags, double* dpnorm, double* times_ms,
+                        int nthreads, int interp);
 int orc_icgn_f32(const uint8_t* ref, const uint8_t* tar, int stride,
                  int img_w, int img_h, int start_x, int start_y,
                  int roi_w, int roi_h, int sx, int sy, int max_iter,

@@ -273,6 +273,71 @@ static int SFX(bicubic_lut)(const uint8_t* img, int stride, int img_w, int img_h
     return 0;
 }
 
+/* F2: BicubicSplineCoefficients_s (TW_utils.cpp:521-602): per ROI pixel,
+ * beta = CP . Omega^T . CP^T from the 4x4 neighbourhood Omega (rows y-1..y+2,
+ * columns x-1..x+2), then B . beta^T . B^T, stored rotated by 180 degrees
+ * (element 4k+l multiplies y^k x^l, as the bicubic LUT).  Every ROI pixel,
+ * the last row and column included, gets coefficients.  Returns -1 when the
+ * neighbourhood of the last row/column would leave the image (the
+ * reference's check at :529-533 admits one row/column past it). */
+static const double SFX(bspline_cp)[4][4] = {
+    {  71 / 56.0, -19 / 56.0,   5 / 56.0,  -1 / 56.0 },
+    { -19 / 56.0,  95 / 56.0, -25 / 56.0,   5 / 56.0 },
+    {   5 / 56.0, -25 / 56.0,  95 / 56.0, -19 / 56.0 },
+    {  -1 / 56.0,   5 / 56.0, -19 / 56.0,  71 / 56.0 }};
+static const double SFX(bspline_base)[4][4] = {
+    { -1 / 6.0,  3 / 6.0, -3 / 6.0, 1 / 6.0 },
+    {  3 / 6.0, -6 / 6.0,  3 / 6.0,       0 },
+    { -3 / 6.0,        0,  3 / 6.0,       0 },
+    {  1 / 6.0,  4 / 6.0,  1 / 6.0,       0 }};
+
+static int SFX(bspline_lut)(const uint8_t* img, int stride, int img_w, int img_h,
+                            int sx0, int sy0, int rw, int rh, REAL* lut)
+{
+    if (sx0 < 1 || sy0 < 1 || img_h - (rh + sy0 + 2) < 0 || img_w - (rw + sx0 + 2) < 0) return -1;
+    REAL cp[4][4], bb[4][4];
+    for (int k = 0; k < 4; ++k)
+        for (int l = 0; l < 4; ++l) {
+            cp[k][l] = (REAL)SFX(bspline_cp)[k][l];
+            bb[k][l] = (REAL)SFX(bspline_base)[k][l];
+        }
+    for (int i = 0; i < rh; ++i)
+        for (int j = 0; j < rw; ++j) {
+            REAL om[4][4], beta[4][4], c[4][4];
+            for (int k = 0; k < 4; ++k)
+                for (int l = 0; l < 4; ++l)
+                    om[k][l] = (REAL)img[(size_t)(i + sy0 - 1 + k) * stride + (j + sx0 - 1 + l)];
+            for (int k = 0; k < 4; ++k)
+                for (int l = 0; l < 4; ++l) {
+                    beta[k][l] = 0;
+                    for (int m = 0; m < 4; ++m)
+                        for (int n = 0; n < 4; ++n) beta[k][l] += cp[k][m] * cp[l][n] * om[n][m];
+                }
+            for (int k = 0; k < 4; ++k)
+                for (int l = 0; l < 4; ++l) {
+                    c[k][l] = 0;
+                    for (int m = 0; m < 4; ++m)
+                        for (int n = 0; n < 4; ++n) c[k][l] += bb[k][m] * bb[l][n] * beta[n][m];
+                }
+            for (int k = 0; k < 2; ++k)                     /* :591-599 */
+                for (int l = 0; l < 4; ++l) {
+                    REAL t = c[k][l];
+                    c[k][l] = c[3 - k][3 - l];
+                    c[3 - k][3 - l] = t;
+                }
+            REAL* o = lut + ((size_t)i * rw + j) * 16;
+            for (int k = 0; k < 4; ++k)
+                for (int l = 0; l < 4; ++l) o[4 * k + l] = c[k][l];
+        }
+    return 0;
+}
+
+int SFX(orc_bspline_lut)(const uint8_t* img, int stride, int img_w, int img_h,
+                         int start_x, int start_y, int roi_w, int roi_h, REAL* lut)
+{
+    return SFX(bspline_lut)(img, stride, img_w, img_h, start_x, start_y, roi_w, roi_h, lut);
+}
+
 /* ---------------------------------------------------------------------- */
 /* A14: ICGN2D_CPU::ICGN2D_Compute (TW_paDIC_ICGN2D_CPU.cpp:481-925).     */
 /* ---------------------------------------------------------------------- */
@@ -435,6 +500,14 @@ static int SFX(icgn_poi)(const SFX(icgn_ctx)* c, REAL* fU, REAL* fV, int32_t* it
     return ORC_SUCCESS;
 }
 
+int SFX(orc_icgn_interp)(const uint8_t* ref, const uint8_t* tar, int stride,
+                         int img_w, int img_h, int start_x, int start_y,
+                         int roi_w, int roi_h, int sx, int sy, int max_iter,
+                         REAL delta_p, const int32_t* poi_yx, int n,
+                         REAL* u, REAL* v, REAL* p6, int32_t* iters,
+                         uint8_t* flags, double* dpnorm, double* times_ms,
+                         int nthreads, int interp);
+
 int SFX(orc_icgn)(const uint8_t* ref, const uint8_t* tar, int stride,
                   int img_w, int img_h, int start_x, int start_y,
                   int roi_w, int roi_h, int sx, int sy, int max_iter,
@@ -442,6 +515,22 @@ int SFX(orc_icgn)(const uint8_t* ref, const uint8_t* tar, int stride,
                   REAL* u, REAL* v, REAL* p6, int32_t* iters,
                   uint8_t* flags, double* dpnorm, double* times_ms,
                   int nthreads)
+{
+    return SFX(orc_icgn_interp)(ref, tar, stride, img_w, img_h, start_x, start_y, roi_w, roi_h, sx, sy,
+                                max_iter, delta_p, poi_yx, n, u, v, p6, iters, flags, dpnorm, times_ms,
+                                nthreads, 0);
+}
+
+/* interp: 0 = Bicubic (TW_paDIC_ICGN2D_CPU.cpp:287-318), 1 = BicubicSpline
+ * (:322-341): same reference gradients, the target LUT from
+ * BicubicSplineCoefficients_s. */
+int SFX(orc_icgn_interp)(const uint8_t* ref, const uint8_t* tar, int stride,
+                         int img_w, int img_h, int start_x, int start_y,
+                         int roi_w, int roi_h, int sx, int sy, int max_iter,
+                         REAL delta_p, const int32_t* poi_yx, int n,
+                         REAL* u, REAL* v, REAL* p6, int32_t* iters,
+                         uint8_t* flags, double* dpnorm, double* times_ms,
+                         int nthreads, int interp)
 {
     if (roi_w < 2 || roi_h < 2 || sx < 0 || sy < 0) return -1;
     if (nthreads > 0) omp_set_num_threads(nthreads);
@@ -454,9 +543,12 @@ int SFX(orc_icgn)(const uint8_t* ref, const uint8_t* tar, int stride,
     REAL* lut = (REAL*)calloc(P * 16, sizeof(REAL));
     int rc = 0;
     double t0 = omp_get_wtime();
-    if (SFX(gradients)(ref, stride, img_w, img_h, start_x, start_y, roi_w, roi_h, rx, ry, NULL) ||
-        SFX(gradients)(tar, stride, img_w, img_h, start_x, start_y, roi_w, roi_h, tx, ty, txy) ||
-        SFX(bicubic_lut)(tar, stride, img_w, img_h, tx, ty, txy, start_x, start_y, roi_w, roi_h, lut))
+    if (SFX(gradients)(ref, stride, img_w, img_h, start_x, start_y, roi_w, roi_h, rx, ry, NULL))
+        rc = -1;
+    else if (interp == 1)
+        rc = SFX(bspline_lut)(tar, stride, img_w, img_h, start_x, start_y, roi_w, roi_h, lut) ? -1 : 0;
+    else if (SFX(gradients)(tar, stride, img_w, img_h, start_x, start_y, roi_w, roi_h, tx, ty, txy) ||
+             SFX(bicubic_lut)(tar, stride, img_w, img_h, tx, ty, txy, start_x, start_y, roi_w, roi_h, lut))
         rc = -1;
     double t1 = omp_get_wtime();
     if (rc == 0) {

@@ -40,12 +40,13 @@ def run_icgn(ctx, poi, u0, v0):
                 dpnorm=dn.download())
 
 
-def check_icgn(g, o, delta_p=0.001, utol=1e-4, gtol=1e-6, band=0.05):
+def check_icgn(g, o, delta_p=0.001, utol=1e-4, gtol=1e-6, band=0.05, max_flips=None):
     """Parity with the oracle o.  Iteration counts may differ only where a
     convergence decision sat inside the FP32 noise band of the loop test: the
     side that stopped first saw |dp| within `band` (relative) of delta_p.  The
     FP32 CPU path carries ~1 % rounding error in |dp| at that point (measured
-    against the FP64 oracle), so this is the reference's own noise."""
+    against the FP64 oracle), so this is the reference's own noise.  At most
+    max_flips such POIs (default max(2, 0.2 % of the POIs))."""
     assert np.array_equal(g["flags"], o["flags"]), np.nonzero(g["flags"] != o["flags"])
     same = g["iters"] == o["iters"]
     flipped = np.nonzero(~same)[0]
@@ -57,7 +58,8 @@ def check_icgn(g, o, delta_p=0.001, utol=1e-4, gtol=1e-6, band=0.05):
         else:
             continue        # GPU stopped first but its |dp| was not returned
         assert abs(stopper / delta_p - 1.0) < band, (i, g["iters"][i], o["iters"][i], stopper)
-    assert len(flipped) <= max(2, int(0.002 * len(same)))
+    cap = max(2, int(0.002 * len(same))) if max_flips is None else max_flips
+    assert len(flipped) <= cap, f"{len(flipped)} iteration-count flips, at most {cap} allowed"
     ok = same & (o["flags"] == oracle.SUCCESS)
     du = max(np.max(np.abs(g["u"][ok] - o["u"][ok])), np.max(np.abs(g["v"][ok] - o["v"][ok])))
     dg = np.max(np.abs(g["p6"][ok][:, [1, 2, 4, 5]] - o["p6"][ok][:, [1, 2, 4, 5]]))

@@ -18,7 +18,7 @@ OK, EINVAL, ENOMEM, EBOUNDS, EHIP, EBUSY, ENODEV = range(7)
 # ICGN2DFlag
 ICGN_SUCCESS, ICGN_DARK_SUBSET, ICGN_SINGULAR_HESSIAN, ICGN_SINGULAR_WARP, ICGN_OUT_OF_ROI = range(5)
 FFTCC_OK, FFTCC_FLAT, FFTCC_OUT = range(3)
-INTERP_LUT, INTERP_FUSED = 0, 1
+INTERP_LUT, INTERP_FUSED, INTERP_BSPLINE = 0, 1, 2
 BUF_RX, BUF_RY, BUF_TX, BUF_TY, BUF_TXY, BUF_LUT, BUF_REF, BUF_TAR = range(8)
 DEFORM_AFFINE, DEFORM_SINUSOID, DEFORM_RESAMPLE = range(3)
 

@@ -282,8 +282,11 @@ cuICGN2D::cuICGN2D(int_t iImgWidth, int_t iImgHeight, int_t iStartX, int_t iStar
              iNumIterations, fDeltaP),
       m_Iflag(Iflag)
 {
-    if (Iflag != ICGN2DInterpolationFLag::Bicubic)   // the reference's GPU B-spline is unfinished too
-        throw std::invalid_argument("cuICGN2D: only ICGN2DInterpolationFLag::Bicubic is implemented on the GPU");
+    // Bicubic: the LUT gather or the fused evaluation (setInterpolationSource);
+    // BicubicSpline: the B-spline LUT of the reference's CPU path
+    // (TW_paDIC_ICGN2D_CPU.cpp:322-341; its GPU version is unfinished)
+    if (Iflag != ICGN2DInterpolationFLag::Bicubic && Iflag != ICGN2DInterpolationFLag::BicubicSpline)
+        throw std::invalid_argument("cuICGN2D: unknown interpolation flag");
     prepare();
 }
 
@@ -298,7 +301,7 @@ void cuICGN2D::prepare()
     c.gx = 1; c.gy = 1; c.mx = 0; c.my = 0;     // POIs come from the caller
     c.fft_n = 2;
     c.max_iter = m_iNumIterations; c.delta_p = m_fDeltaP;
-    c.interp = m_source;
+    c.interp = m_Iflag == ICGN2DInterpolationFLag::BicubicSpline ? TWDIC_INTERP_BSPLINE : m_source;
     check(twdic_create(&c, 0, &m_ctx), nullptr, "twdic_create");
     const size_t n = (size_t)m_iPOINumber;
     g_cuHandleICGN.m_d_iIterationNums = dev_alloc<int>(m_ctx, n);

@@ -169,14 +169,15 @@ __device__ __forceinline__ bool subset_loop(const IcgnParams& P, int px, int py,
         bool in = true, interior = true;
         if (CHECK) {
             in = ix >= sx0 && iy >= sy0 && ix < sx0 + rw && iy < sy0 + rh;
-            interior = in && ix != sx0 + rw - 1 && iy != sy0 + rh - 1;   // the CPU LUT's last row/column is zero
+            // the CPU bicubic LUT's last row/column is zero; the B-spline LUT covers it
+            interior = in && (INTERP == TWDIC_INTERP_BSPLINE || (ix != sx0 + rw - 1 && iy != sy0 + rh - 1));
             out |= !in;
         }
         if (INT) {
             if (in) T = (float)P.tar[(uint32_t)(iy * P.stride + ix)];              // :629
         } else if (interior) {
             const float tx = wx - (float)ix, ty = wy - (float)iy;                // :771-772
-            if (INTERP == TWDIC_INTERP_LUT)
+            if (INTERP != TWDIC_INTERP_FUSED)
                 T = interp_lut(P.lut, (uint32_t)((iy - sy0) * rw + (ix - sx0)), tx, ty);
             else
                 T = interp_fused(P.tar, P.stride, ix, iy, tx, ty);
@@ -380,6 +381,8 @@ hipError_t launch_icgn(const IcgnParams& p, int interp, int variant, hipStream_t
     dim3 grid((p.n + 3) / 4);
     if (interp == TWDIC_INTERP_FUSED)
         launch_icgn_t<TWDIC_INTERP_FUSED>(p, variant, grid, st);
+    else if (interp == TWDIC_INTERP_BSPLINE)
+        launch_icgn_t<TWDIC_INTERP_BSPLINE>(p, variant, grid, st);
     else
         launch_icgn_t<TWDIC_INTERP_LUT>(p, variant, grid, st);
     return hipGetLastError();

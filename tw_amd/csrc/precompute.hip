@@ -67,6 +67,82 @@ hipError_t launch_gradients(const uint8_t* img, int stride, int sx0, int sy0, in
 }
 
 // ---------------------------------------------------------------------------
+// Cubic B-spline LUT: BicubicSplineCoefficients_s (TW_utils.cpp:521-602).
+// From the 4x4 neighbourhood Om (rows y-1..y+2, columns x-1..x+2):
+// beta = CP Om^T CP^T with the local prefilter CP, then B beta^T B^T with the
+// B-spline basis B, rotated by 180 degrees.  FP32, every product and sum
+// rounded separately in the reference's order.  Every ROI pixel gets
+// coefficients (the caller guarantees a 2-pixel frame right and below).
+// ---------------------------------------------------------------------------
+namespace {
+__device__ __forceinline__ float bsp_cp(int k, int l)
+{
+    constexpr float t[4][4] = {{(float)(71 / 56.0), (float)(-19 / 56.0), (float)(5 / 56.0), (float)(-1 / 56.0)},
+                               {(float)(-19 / 56.0), (float)(95 / 56.0), (float)(-25 / 56.0), (float)(5 / 56.0)},
+                               {(float)(5 / 56.0), (float)(-25 / 56.0), (float)(95 / 56.0), (float)(-19 / 56.0)},
+                               {(float)(-1 / 56.0), (float)(5 / 56.0), (float)(-19 / 56.0), (float)(71 / 56.0)}};
+    return t[k][l];
+}
+__device__ __forceinline__ float bsp_base(int k, int l)
+{
+    constexpr float t[4][4] = {{(float)(-1 / 6.0), (float)(3 / 6.0), (float)(-3 / 6.0), (float)(1 / 6.0)},
+                               {(float)(3 / 6.0), (float)(-6 / 6.0), (float)(3 / 6.0), 0.f},
+                               {(float)(-3 / 6.0), 0.f, (float)(3 / 6.0), 0.f},
+                               {(float)(1 / 6.0), (float)(4 / 6.0), (float)(1 / 6.0), 0.f}};
+    return t[k][l];
+}
+}  // namespace
+
+__global__ __launch_bounds__(256) void k_bspline_lut(const uint8_t* __restrict__ img, int stride, int sx0, int sy0,
+                                                     int rw, int rh, float* __restrict__ lut)
+{
+#pragma clang fp contract(off)
+    const int x = blockIdx.x * 64 + (threadIdx.x & 63);
+    const int y = blockIdx.y * 4 + (threadIdx.x >> 6);
+    if (x >= rw || y >= rh) return;
+    float om[4][4], beta[4][4], c[4][4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+#pragma unroll
+        for (int l = 0; l < 4; ++l)
+            om[k][l] = (float)img[(size_t)(y + sy0 - 1 + k) * stride + (x + sx0 - 1 + l)];
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+#pragma unroll
+        for (int l = 0; l < 4; ++l) {
+            float b = 0.f;
+#pragma unroll
+            for (int m = 0; m < 4; ++m)
+#pragma unroll
+                for (int n = 0; n < 4; ++n) b = b + bsp_cp(k, m) * bsp_cp(l, n) * om[n][m];
+            beta[k][l] = b;
+        }
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+#pragma unroll
+        for (int l = 0; l < 4; ++l) {
+            float a = 0.f;
+#pragma unroll
+            for (int m = 0; m < 4; ++m)
+#pragma unroll
+                for (int n = 0; n < 4; ++n) a = a + bsp_base(k, m) * bsp_base(l, n) * beta[n][m];
+            c[k][l] = a;
+        }
+    float4* out = reinterpret_cast<float4*>(lut + ((size_t)y * rw + x) * 16);
+#pragma unroll
+    for (int k = 0; k < 4; ++k)      // rotated by 180 degrees (:591-599)
+        out[k] = make_float4(c[3 - k][3], c[3 - k][2], c[3 - k][1], c[3 - k][0]);
+}
+
+hipError_t launch_bspline_lut(const uint8_t* tar, int stride, int sx0, int sy0, int rw, int rh, float* lut,
+                              hipStream_t st)
+{
+    dim3 grid((rw + 63) / 64, (rh + 3) / 4);
+    hipLaunchKernelGGL(k_bspline_lut, grid, dim3(256), 0, st, tar, stride, sx0, sy0, rw, rh, lut);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
 // Bicubic LUT: BicubicCoefficients_s (TW_utils.cpp:604-700), fused with the
 // target gradients: tau (corner values, x/y/xy central differences) is built
 // straight from the 4x4 pixel neighbourhood, alpha = M tau with the standard

@@ -56,7 +56,11 @@ const char* validate(const twdic_config& c)
     if (c.fft_n < 2 || c.fft_n > kMaxFftN || (c.fft_n & 1)) return "fft_n must be even and in [2, 64]";
     if (c.max_iter < 1) return "max_iter must be >= 1";
     if (!(c.delta_p >= 0.f)) return "delta_p must be >= 0";
-    if (c.interp != TWDIC_INTERP_LUT && c.interp != TWDIC_INTERP_FUSED) return "unknown interp";
+    if (c.interp != TWDIC_INTERP_LUT && c.interp != TWDIC_INTERP_FUSED && c.interp != TWDIC_INTERP_BSPLINE)
+        return "unknown interp";
+    if (c.interp == TWDIC_INTERP_BSPLINE &&
+        (c.roi_x < 1 || c.roi_y < 1 || c.roi_x + c.roi_w + 2 > c.img_w || c.roi_y + c.roi_h + 2 > c.img_h))
+        return "the B-spline LUT needs a 1-pixel frame left/above and 2 pixels right/below the ROI";
     int nx, ny;
     num_poi(c, &nx, &ny);
     if (nx <= 0 || ny <= 0) return "Number of POIs is below 0!";
@@ -284,7 +288,10 @@ twdic_status twdic_precompute_target(twdic_ctx* c)
     const twdic_config& g = c->cfg;
     TW_HIP(c, launch_gradients(c->d_tar, c->dstride, g.roi_x, g.roi_y, g.roi_w, g.roi_h, c->d_tx, c->d_ty,
                                c->d_txy, nullptr, c->stream));
-    TW_HIP(c, launch_bicubic_lut(c->d_tar, c->dstride, g.roi_x, g.roi_y, g.roi_w, g.roi_h, c->d_lut, c->stream));
+    if (g.interp == TWDIC_INTERP_BSPLINE)
+        TW_HIP(c, launch_bspline_lut(c->d_tar, c->dstride, g.roi_x, g.roi_y, g.roi_w, g.roi_h, c->d_lut, c->stream));
+    else
+        TW_HIP(c, launch_bicubic_lut(c->d_tar, c->dstride, g.roi_x, g.roi_y, g.roi_w, g.roi_h, c->d_lut, c->stream));
     c->tar_pre_valid = true;
     return TWDIC_OK;
 }
@@ -301,6 +308,10 @@ twdic_status twdic_icgn(twdic_ctx* c, const int32_t* d_poi, int n, float* d_u, f
     if (g.interp == TWDIC_INTERP_LUT && !c->tar_pre_valid) {
         // per-frame target LUT (cuBicubicCoefficients, TW_paDIC_cuICGN2D.cu:844-851)
         TW_HIP(c, launch_bicubic_lut(c->d_tar, c->dstride, g.roi_x, g.roi_y, g.roi_w, g.roi_h, c->d_lut, c->stream));
+        c->tar_pre_valid = true;
+    } else if (g.interp == TWDIC_INTERP_BSPLINE && !c->tar_pre_valid) {
+        // per-frame B-spline LUT (BicubicSplineCoefficients_s, TW_paDIC_ICGN2D_CPU.cpp:333-340)
+        TW_HIP(c, launch_bspline_lut(c->d_tar, c->dstride, g.roi_x, g.roi_y, g.roi_w, g.roi_h, c->d_lut, c->stream));
         c->tar_pre_valid = true;
     }
     if ((size_t)n > c->refstate_cap) {

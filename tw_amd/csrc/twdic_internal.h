@@ -104,6 +104,8 @@ __device__ __forceinline__ ConstPairDescPtr const_pairs(const PairDesc* p)
 // Kernel launch wrappers (return hipError_t of the launch).
 hipError_t launch_gradients(const uint8_t* img, int stride, int sx0, int sy0, int rw, int rh,
                             float* gx, float* gy, float* gxy, uint32_t* g2, hipStream_t st);
+hipError_t launch_bspline_lut(const uint8_t* tar, int stride, int sx0, int sy0, int rw, int rh, float* lut,
+                              hipStream_t st);
 hipError_t launch_bicubic_lut(const uint8_t* tar, int stride, int sx0, int sy0, int rw, int rh,
                               float* lut, hipStream_t st);
 hipError_t launch_poi_grid(int32_t* poi, int start_x, int start_y, int nx, int ny, int mx, int my,
