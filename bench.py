@@ -32,6 +32,7 @@ sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBS = 8000.0
 FP32_PEAK_TFLOPS = 157.3
+INT8_MFMA_PEAK_TOPS = 5000.0
 
 
 def measured_traffic() -> dict:
@@ -276,18 +277,23 @@ def main():
     kbar = float(git[ok].mean())
 
     # Roofline of each kernel from its measured launch time; the dominant
-    # (longer) one is reported as "roofline".  FLOP conventions: F_cc(N) =
-    # 3 * 2.5 N^2 log2(N^2) + 6 N (N/2 + 1) + 7 N^2 per POI (two forward and
-    # one inverse N x N transform per POI, spectrum product, moments and peak),
-    # F_it(s) = 66 s per IC-GN iteration over an s-pixel subset.
+    # (longer) one is reported as "roofline".  Work conventions:
+    #   FFT-CC on the matrix cores (default engine): the whole integer
+    #   correlation surface, 2 N^4 int8 operations per POI, against the dense
+    #   int8 MFMA peak; the FFT formulation's F_cc(N) = 3 * 2.5 N^2 log2(N^2)
+    #   + 6 N (N/2 + 1) + 7 N^2 (two forward and one inverse N x N transform,
+    #   spectrum product, moments and peak) is reported beside it, and is the
+    #   measure for the FP32 FFT engine;
+    #   IC-GN: F_it(s) = 66 s FP32 operations per iteration over an s-pixel subset.
     s = (2 * case.S + 1) ** 2
     N = case.N
     P = case.roi[2] * case.roi[3]
     icgn_ms = float(np.mean(t_icgn))
     fft_ms = float(np.mean(t_fft))
     f_cc = 3 * 2.5 * N * N * np.log2(N * N) + 6 * N * (N // 2 + 1) + 7 * N * N
+    mma = os.environ.get("TWDIC_FFTCC_VARIANT", "8") not in ("2", "3", "4")
     kern = {
-        "k_fftcc": {"ms": fft_ms, "flops": f_cc * n,
+        "k_fftcc": {"ms": fft_ms, "flops": f_cc * n, "int8_ops": 2.0 * N ** 4 * n if mma else None,
                     # both images read once, POI in, u/v/ZNCC out
                     "bytes": 2 * W * H + n * (8 + 12)},
         "k_icgn": {"ms": icgn_ms, "flops": 66.0 * s * kbar * n,
@@ -305,6 +311,16 @@ def main():
     D = kern[dom]
 
     def roof(name, d):
+        if d.get("int8_ops"):
+            tops = d["int8_ops"] / (d["ms"] * 1e-3) / 1e12
+            return {"bound": "mfma", "pipe": "int8 MFMA (v_mfma_i32_32x32x32_i8), exact integer correlation",
+                    "kernel": name, "achieved": tops, "peak": INT8_MFMA_PEAK_TOPS, "unit": "TOP/s",
+                    "frac": tops / INT8_MFMA_PEAK_TOPS, "traffic": d["traffic"],
+                    "avg_launch_ms": d["ms"], "algorithmic_ops_per_launch": d["int8_ops"],
+                    "fft_equivalent": {"achieved": d["tflops"], "unit": "TFLOP/s",
+                                       "algorithmic_flops_per_launch": d["flops"]},
+                    "hbm": {"achieved": d["gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                            "frac": d["gbs"] / HBM_PEAK_GBS, "algorithmic_bytes_per_launch": d["bytes"]}}
         return {"bound": "mfma", "pipe": "fp32 VALU (no MFMA: no dense contraction on this path)",
                 "kernel": name, "achieved": d["tflops"], "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
                 "frac": d["tflops"] / FP32_PEAK_TFLOPS, "traffic": d["traffic"],
@@ -333,6 +349,7 @@ def main():
         "fftcc_pois_per_s": n / (fft_ms * 1e-3),
         "latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99))},
         "kernel_ms": {"fftcc": fft_ms, "icgn": icgn_ms},
+        "fftcc_engine": "int8 matrix cores, exact" if mma else "fp32 FFT with exact re-evaluation",
         "mean_iterations": kbar,
         "flags": np.bincount(gfl, minlength=5).tolist(),
         "subpixel_err_px": {"median": float(np.median(err)), "p99": float(np.percentile(err, 99)),

@@ -55,8 +55,9 @@ def test_gradients_and_lut_bit_exact(c512):
 # ---------------------------------------------------------------------- FFT-CC
 @pytest.mark.parametrize("N", [32, 30, 20, 24])
 def test_fftcc_peaks_exact(c512, N):
-    """N = 32: radix-4 register engine; N = 30 = 2S: the reference's own window
-    (direct-DFT engine); 20, 24: further generic sizes."""
+    """Default (matrix-core) engine.  N = 32: one full 32 x 32 tile; N = 30 =
+    2S: the reference's own window (padded K and output); 20, 24: further
+    generic sizes."""
     poi = c512.poi()
     with ctx_for(c512, N=N) as ctx:
         g = run_fftcc(ctx, poi)
@@ -74,6 +75,26 @@ def test_fftcc_peaks_exact(c512, N):
     # value; the FP32 CPU path's own FFT rounding moves its ZNCC up to 2.9e-5
     assert np.max(np.abs(g["zncc"][ok] - ex["zncc"][ok])) <= 1e-6
     assert np.max(np.abs(g["zncc"][ok] - f32["zncc"][ok])) <= 3e-5
+
+
+@pytest.mark.parametrize("N", [32, 30, 64])
+def test_fftcc_engines_agree_bitwise(N, monkeypatch):
+    """The matrix-core engine and the FP32 FFT engines (radix-4 register
+    engine for N = 32/64, direct DFT otherwise) return the exact integer peak
+    and the ZNCC evaluated from exact integer sums, so their outputs are
+    bit-identical.  N = 64 runs on the 2048 workload (its POIs leave room for
+    the window), the others on the 512 one."""
+    c = cases.speckle_2048() if N == 64 else cases.speckle_512()
+    poi = c.poi()
+    if N == 64:
+        poi = poi[::7]
+    out = {}
+    for variant in ("8", "3"):
+        monkeypatch.setenv("TWDIC_FFTCC_VARIANT", variant)
+        with ctx_for(c, N=N) as ctx:
+            out[variant] = run_fftcc(ctx, poi)
+    for k in ("u", "v", "zncc", "status"):
+        assert np.array_equal(out["8"][k], out["3"][k]), k
 
 
 def test_fftcc_shift_recovery():

@@ -9,27 +9,32 @@
 //
 // Exactness: the zero-mean correlation differs from the integer correlation
 // I[k] = sum_x R(x) T(x+k) by a constant, so the true peak is an integer
-// argmax.  The FP32 transforms give the maximum; when another shift lies
-// within `tol` (ZNCC units) of it, all such shifts are re-evaluated exactly in
-// integers (lowest index wins ties).  With a single candidate the FP32 argmax
-// is provably the exact one.  The reported ZNCC is evaluated in FP64 from the
-// exact integer sums at the chosen peak, so every output is a function of the
-// two windows alone: results do not depend on batching or pairing.
+// argmax, and every engine here returns that exact peak.  The reported ZNCC
+// is evaluated in FP64 from the exact integer sums at the chosen peak, so
+// every output is a function of the two windows alone: results do not depend
+// on the engine, batching or pairing.
 //
-// Two transform engines:
-//   * k_fftcc_fft<P> (N = 4P = 32 or 64): one workgroup of 4N threads owns a
-//     POI pair.  Each window pair is packed into Z = R^ + i T^ (R^, T^
-//     zero-mean, unit norm), so one forward 2-D FFT yields both spectra; the
-//     cross-power spectra X_A, X_B are Hermitian, so a single inverse FFT of
-//     X_A + i X_B returns both correlation surfaces (real and imaginary
-//     parts): 3 complex transforms per 2 POIs.  A 4N-point line transform
-//     runs as a P-point radix-4 DFT in registers and a 4-point DFT across a
-//     lane quad with DPP quad permutes; transposes go through LDS.
-//   * k_fftcc_dft: direct DFT along rows/columns for any other even N <= 64
-//     (the reference's own window N = 2S = 20, 30, 40, 60).
+// Engines:
+//   * k_fftcc_mma<NT> (default, any even N <= 64): the whole integer
+//     correlation surface on the int8 matrix cores, one wavefront per POI.
+//   * k_fftcc_fft<P> (N = 4P = 32 or 64): FP32 FFTs.  One workgroup of 4N
+//     threads owns a POI pair.  Each window pair is packed into
+//     Z = R^ + i T^ (R^, T^ zero-mean, unit norm), so one forward 2-D FFT
+//     yields both spectra; the cross-power spectra X_A, X_B are Hermitian, so
+//     a single inverse FFT of X_A + i X_B returns both correlation surfaces
+//     (real and imaginary parts): 3 complex transforms per 2 POIs.  A 4N-point
+//     line transform runs as a P-point radix-4 DFT in registers and a 4-point
+//     DFT across a lane quad with DPP quad permutes; transposes go through
+//     LDS.  When another shift lies within `tol` (ZNCC units) of the FP32
+//     maximum, all such shifts are re-evaluated exactly in integers (lowest
+//     index wins ties); with a single candidate the FP32 argmax is provably
+//     the exact one.
+//   * k_fftcc_dft: FP32 direct DFT along rows/columns for any other even
+//     N <= 64, with the same exact re-evaluation.
 #include "twdic_internal.h"
 
 #include <cfloat>
+#include <climits>
 
 namespace twdic {
 
@@ -728,13 +733,269 @@ __global__ __launch_bounds__(kDftThreads) void k_batch_fftcc_dft(const PairDesc*
     }
 }
 
+// ===========================================================================
+// Matrix-core engine: any even N <= 64, one wavefront per POI.
+//
+// The inverse transform of the cross-power spectrum (:285-297) is the
+// circular cross-correlation C[kv][ku] = sum_{y,x} R[y][x] T[(y+kv)%N][(x+ku)%N],
+// up to the zero-mean constant and rounding.  This engine computes it
+// directly, exactly, in integers on the int8 matrix cores
+// (v_mfma_i32_32x32x32_i8).  Stored as signed bytes R' = R - 128 and
+// T' = T - 128, C'[k] = C[k] - 128 (sum R + sum T) + 128^2 N^2 differs from
+// C[k] by a constant, so the argmax (lowest row-major index on ties) is the
+// exact integer peak, and |C'| <= 2^14 N^2 <= 2^26 fits int32.
+//
+// GEMM form, K = (y', x) over target rows y' and columns x:
+//   C'[kv][ku] = sum_{y'} sum_x R'[(y' - kv) mod N][x] * T'[y'][(x + ku) mod N]
+// A[kv][(y', x)] gathers reference rows (the row index turns with y');
+// B[(y', x)][ku] is the circulant of target row y', read from a doubled copy
+// of the row, T2[c] = T'[c mod N], at byte offset x + ku.  Outputs are
+// NT x NT (NT = 32 for N <= 32, else 64) in 32 x 32 tiles; rows and columns
+// of K or of the output beyond N are zero padding or ignored.
+//
+// Lane layout (one wavefront, lane l, m = l % 32, h = l / 32): an A operand
+// holds row m, bytes 16h .. 16h+15 of a 32-byte K chunk; a B operand holds
+// column m, the same 16 K bytes; accumulator register r of lane l is output
+// (row 8 (r / 4) + 4 h + r % 4, column m) of the tile.
+// ===========================================================================
+namespace {
+
+typedef int v4i __attribute__((ext_vector_type(4)));
+typedef int v16i __attribute__((ext_vector_type(16)));
+
+// Row strides are multiples of 16 bytes whose dword counts put 8 consecutive
+// rows in distinct 4-bank groups, so the row-per-lane 16-byte stores and the
+// reference-row reads are conflict-free.
+template <int NT, bool N64>
+struct MmaSmem {
+    static constexpr int RS = NT == 64 ? 80 : 48;     // reference row stride
+    static constexpr int TW = N64 ? 96 : 2 * NT;      // bytes of the doubled target row that are read
+    static constexpr int TS = N64 ? 112 : NT == 64 ? 144 : 80;   // target row stride
+    alignas(16) uint8_t r[NT * RS];
+    alignas(16) uint8_t t[NT * TS];
+};
+
+// lane l takes lane l - 1's value (lane 0 keeps its own)
+__device__ __forceinline__ v4i lane_shift_up(v4i a)
+{
+    v4i b;
+    b.x = __builtin_amdgcn_update_dpp(a.x, a.x, 0x138, 0xf, 0xf, false);   // wave_shr:1
+    b.y = __builtin_amdgcn_update_dpp(a.y, a.y, 0x138, 0xf, 0xf, false);
+    b.z = __builtin_amdgcn_update_dpp(a.z, a.z, 0x138, 0xf, 0xf, false);
+    b.w = __builtin_amdgcn_update_dpp(a.w, a.w, 0x138, 0xf, 0xf, false);
+    return b;
+}
+
+// 16 bytes of a doubled target row starting at byte offset o (any alignment)
+__device__ __forceinline__ v4i b_operand(const uint32_t* row, int d, int sh)
+{
+    const uint32_t w0 = row[d], w1 = row[d + 1], w2 = row[d + 2], w3 = row[d + 3], w4 = row[d + 4];
+    v4i b;
+    b.x = (int)__builtin_amdgcn_alignbyte(w1, w0, sh);
+    b.y = (int)__builtin_amdgcn_alignbyte(w2, w1, sh);
+    b.z = (int)__builtin_amdgcn_alignbyte(w3, w2, sh);
+    b.w = (int)__builtin_amdgcn_alignbyte(w4, w3, sh);
+    return b;
+}
+
+// N64: N == 64 (NT = 64); otherwise N <= NT is a run-time value
+template <int NT, bool N64>
+__global__ __launch_bounds__(64) void k_fftcc_mma(FftccParams p)
+{
+    constexpr int TT = NT / 32;          // output tiles per edge and K chunks per row
+    using Sm = MmaSmem<NT, N64>;
+    __shared__ Sm sm;
+    const int poi = blockIdx.x;
+    const int lane = threadIdx.x, m = lane & 31, h = lane >> 5;
+    const int N = N64 ? 64 : p.N, NN = N * N;
+    const int py = p.poi[2 * poi], px = p.poi[2 * poi + 1];
+    if (py - N / 2 < 0 || px - N / 2 < 0 || py + N / 2 > p.img_h || px + N / 2 > p.img_w) {
+        if (lane == 0 && p.status) p.status[poi] = TWDIC_FFTCC_OUT;
+        return;
+    }
+
+    // -- window rows to LDS (lane = row), exact integer sums -------------------
+    int sr = 0, st = 0, srr = 0, stt = 0;
+    if (lane < N) {
+        const size_t o = (size_t)(py - N / 2 + lane) * p.stride + (px - N / 2);
+        uint8_t* rrow = sm.r + lane * Sm::RS;
+        uint8_t* trow = sm.t + lane * Sm::TS;
+        if (N == NT) {
+            uint32_t r[NT / 4], t[NT / 4];
+            __builtin_memcpy(r, p.ref + o, NT);
+            __builtin_memcpy(t, p.tar + o, NT);
+#pragma unroll
+            for (int k = 0; k < NT / 4; ++k) {
+                sr = (int)__builtin_amdgcn_udot4(r[k], 0x01010101u, (uint32_t)sr, false);
+                st = (int)__builtin_amdgcn_udot4(t[k], 0x01010101u, (uint32_t)st, false);
+                srr = (int)__builtin_amdgcn_udot4(r[k], r[k], (uint32_t)srr, false);
+                stt = (int)__builtin_amdgcn_udot4(t[k], t[k], (uint32_t)stt, false);
+            }
+            uint4* r4 = reinterpret_cast<uint4*>(rrow);
+            uint4* t4 = reinterpret_cast<uint4*>(trow);
+#pragma unroll
+            for (int k = 0; k < NT / 16; ++k)         // signed bytes
+                r4[k] = make_uint4(r[4 * k] ^ 0x80808080u, r[4 * k + 1] ^ 0x80808080u, r[4 * k + 2] ^ 0x80808080u,
+                                   r[4 * k + 3] ^ 0x80808080u);
+#pragma unroll
+            for (int k = 0; k < Sm::TW / 16; ++k) {   // the target row, repeated
+                const int q = k % (NT / 16);
+                t4[k] = make_uint4(t[4 * q] ^ 0x80808080u, t[4 * q + 1] ^ 0x80808080u, t[4 * q + 2] ^ 0x80808080u,
+                                   t[4 * q + 3] ^ 0x80808080u);
+            }
+        } else {
+            // any even N < NT: bytewise stores only (reference bytes x >= N are
+            // zero, the target row repeats with period N), then the sums from
+            // the signed bytes with explicit dot products.  (A guarded byte sum
+            // `if (x < N) s += a * a` was folded by the compiler into a v_dot4
+            // with a wrong accumulator, adding 1 per row.)
+            for (int x = 0; x < NT; ++x) rrow[x] = x < N ? (uint8_t)(p.ref[o + x] ^ 0x80) : (uint8_t)0;
+            for (int c = 0, x = 0; c < Sm::TW; ++c) {
+                trow[c] = (uint8_t)(p.tar[o + x] ^ 0x80);
+                if (++x == N) x = 0;
+            }
+            const int* r32 = reinterpret_cast<const int*>(rrow);
+            const int* t32 = reinterpret_cast<const int*>(trow);
+            int s1r = 0, s2r = 0, s1t = 0, s2t = 0;
+#pragma unroll
+            for (int k = 0; k < NT / 4; ++k) {
+                const int x = 4 * k;
+                const int keep = x + 4 <= N ? -1 : x >= N ? 0 : (int)((1u << (8 * (N - x))) - 1u);
+                const int wr = r32[k], wt = t32[k] & keep;
+                s1r = __builtin_amdgcn_sdot4(wr, 0x01010101, s1r, false);
+                s2r = __builtin_amdgcn_sdot4(wr, wr, s2r, false);
+                s1t = __builtin_amdgcn_sdot4(wt, 0x01010101, s1t, false);
+                s2t = __builtin_amdgcn_sdot4(wt, wt, s2t, false);
+            }
+            // bytes are v - 128: sum v = s1 + 128 N, sum v^2 = s2 + 256 s1 + 16384 N
+            sr = s1r + 128 * N;
+            srr = s2r + 256 * s1r + 16384 * N;
+            st = s1t + 128 * N;
+            stt = s2t + 256 * s1t + 16384 * N;
+        }
+    }
+    sr = wave_sum_i(sr); st = wave_sum_i(st); srr = wave_sum_i(srr); stt = wave_sum_i(stt);
+    const long long Sr = sr, St = st, Srr = srr, Stt = stt;
+    if (Srr * NN == Sr * Sr || Stt * NN == St * St) {          // flat window (:271-275)
+        if (lane == 0 && p.status) p.status[poi] = TWDIC_FFTCC_FLAT;
+        return;
+    }
+    __syncthreads();
+
+    // -- C' = A B over K = N rows x TT chunks of 32 bytes ------------------------
+    int ra[TT];                                        // A row byte offsets, (y' - kv) mod N
+#pragma unroll
+    for (int t = 0; t < TT; ++t) {
+        const int kv = 32 * t + m;
+        ra[t] = (kv < N ? (kv == 0 ? 0 : N - kv) : 0) * Sm::RS + 16 * h;
+    }
+    // B operand j (j = K chunk + output tile) starts at byte 32 j + 16 h + m
+    constexpr int NB = TT == 1 ? 1 : N64 ? 2 : 3;      // distinct B operands per row
+    int bd[NB], bs[NB];
+#pragma unroll
+    for (int j = 0; j < NB; ++j) {
+        const int o = 32 * j + 16 * h + m;
+        bd[j] = o >> 2;
+        bs[j] = o & 3;
+    }
+    const int rwrap = N * Sm::RS;
+    v16i acc[TT][TT];
+#pragma unroll
+    for (int a = 0; a < TT; ++a)
+#pragma unroll
+        for (int b = 0; b < TT; ++b) acc[a][b] = v16i{};
+
+    // A operands of row y': lane (m, h) holds reference row (y' - kv) mod N.
+    // From one row to the next every lane takes its lower neighbour's value
+    // (kv - 1 at y' is kv at y' + 1); only the m = 0 lanes read a new row.
+    v4i A[TT][TT];                                     // [K chunk][output row tile]
+#pragma unroll
+    for (int xc = 0; xc < TT; ++xc)
+#pragma unroll
+        for (int t = 0; t < TT; ++t) A[xc][t] = *reinterpret_cast<const v4i*>(sm.r + ra[t] + 32 * xc);
+
+#pragma unroll 2
+    for (int y = 0; y < N; ++y) {
+        const uint32_t* trow = reinterpret_cast<const uint32_t*>(sm.t + y * Sm::TS);
+        v4i B[NB];
+#pragma unroll
+        for (int j = 0; j < NB; ++j) B[j] = b_operand(trow, bd[j], bs[j]);
+#pragma unroll
+        for (int xc = 0; xc < TT; ++xc)
+#pragma unroll
+            for (int t = 0; t < TT; ++t)
+#pragma unroll
+                for (int t2 = 0; t2 < TT; ++t2)     // offset 64 is offset 0 when N = 64
+                    acc[t][t2] = __builtin_amdgcn_mfma_i32_32x32x32_i8(A[xc][t], B[(xc + t2) % NB], acc[t][t2], 0, 0, 0);
+#pragma unroll
+        for (int t = 0; t < TT; ++t) {
+            ra[t] += Sm::RS;
+            if (ra[t] >= rwrap) ra[t] -= rwrap;
+        }
+#pragma unroll
+        for (int xc = 0; xc < TT; ++xc)
+#pragma unroll
+            for (int t = 0; t < TT; ++t) A[xc][t] = lane_shift_up(A[xc][t]);
+        if (m == 0) {
+#pragma unroll
+            for (int xc = 0; xc < TT; ++xc)
+#pragma unroll
+                for (int t = 0; t < TT; ++t) A[xc][t] = *reinterpret_cast<const v4i*>(sm.r + ra[t] + 32 * xc);
+        }
+    }
+
+    // -- exact peak: largest C', lowest row-major index on ties (:300-311) -------
+    // a lane's values are visited in increasing index order (kv, then ku), so
+    // a strict comparison keeps the lowest index among equal values
+    int best = INT_MIN, bk = INT_MAX;
+#pragma unroll
+    for (int t = 0; t < TT; ++t)
+#pragma unroll
+        for (int r = 0; r < 16; ++r)
+#pragma unroll
+            for (int t2 = 0; t2 < TT; ++t2) {
+                const int kv = 32 * t + 8 * (r >> 2) + 4 * h + (r & 3), ku = 32 * t2 + m;
+                const int v = acc[t][t2][r];
+                const bool take = kv < N && ku < N && v > best;
+                best = take ? v : best;
+                bk = take ? kv * N + ku : bk;
+            }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        const int ov = __shfl_xor(best, o), ok = __shfl_xor(bk, o);
+        const bool take = ov > best || (ov == best && ok < bk);
+        best = take ? ov : best;
+        bk = take ? ok : bk;
+    }
+    if (lane == 0) {
+        const long long I = (long long)best + 128LL * (Sr + St) - 16384LL * NN;   // sum R T at the peak
+        const double nr = (double)Srr - (double)Sr * (double)Sr / (double)NN;
+        const double nt = (double)Stt - (double)St * (double)St / (double)NN;
+        int iu, iv;
+        wrap_peak(bk, N, iu, iv);
+        p.u[poi] = (float)iu;
+        p.v[poi] = (float)iv;
+        p.zncc[poi] = (float)(((double)I - (double)Sr * (double)St / (double)NN) / sqrt(nr * nt));
+        if (p.status) p.status[poi] = TWDIC_FFTCC_OK;
+    }
+}
+
+}  // namespace
+
 // variant: occupancy target of the register engine (waves per SIMD the
 // compiler budgets registers for): 2 (no spills), 3 or 4.
 hipError_t launch_fftcc(const FftccParams& p, int variant, hipStream_t st)
 {
     if (p.n <= 0) return hipSuccess;
     const int pairs = (p.n + 1) / 2;
-    if (p.N == 64) {
+    if (variant == kFftccMma) {
+        if (p.N == 64)
+            hipLaunchKernelGGL((k_fftcc_mma<64, true>), dim3(p.n), dim3(64), 0, st, p);
+        else if (p.N > 32)
+            hipLaunchKernelGGL((k_fftcc_mma<64, false>), dim3(p.n), dim3(64), 0, st, p);
+        else
+            hipLaunchKernelGGL((k_fftcc_mma<32, false>), dim3(p.n), dim3(64), 0, st, p);
+    } else if (p.N == 64) {
         if (variant == 3)
             hipLaunchKernelGGL((k_fftcc_fft<16, 3>), dim3(pairs), dim3(256), 0, st, p);
         else if (variant == 4)

@@ -110,6 +110,10 @@ hipError_t launch_bicubic_lut(const uint8_t* tar, int stride, int sx0, int sy0, 
                               float* lut, hipStream_t st);
 hipError_t launch_poi_grid(int32_t* poi, int start_x, int start_y, int nx, int ny, int mx, int my,
                            int sx, int sy, int gx, int gy, hipStream_t st);
+// variant kFftccMma: exact integer correlation on the matrix cores (any even
+// N <= 64); 2, 3, 4: the FFT register engine's occupancy target (N = 32, 64,
+// else the direct-DFT engine)
+constexpr int kFftccMma = 8;
 hipError_t launch_fftcc(const FftccParams& p, int variant, hipStream_t st);
 hipError_t launch_ref_state(const IcgnParams& p, hipStream_t st);
 hipError_t launch_icgn(const IcgnParams& p, int interp, int variant, hipStream_t st);
@@ -161,7 +165,7 @@ struct twdic_ctx {
     uint64_t refstate_version = ~0ull;
     const int32_t* refstate_poi = nullptr;
     int refstate_n = -1;
-    int fftcc_variant = 3;            // tuned launch variant of the FFT-CC engine
+    int fftcc_variant = twdic::kFftccMma;  // FFT-CC engine (TWDIC_FFTCC_VARIANT: 8 matrix cores; 2-4 FFT)
     int icgn_variant = 5;             // occupancy target of the IC-GN kernel (0: compiler's choice)
     uint8_t* h_stage = nullptr;       // pinned staging for host images
     size_t h_stage_bytes = 0;
