@@ -34,10 +34,12 @@ __device__ __forceinline__ double wave_sum(double v)
 }
 
 // Warp coordinate exactly as the CPU path evaluates it (:618-619, :761-762):
-// ((POI + W0*dx) + W1*dy) + W2, every product and sum rounded to FP32.
-__device__ __forceinline__ float warp_coord(int poi, float w0, float w1, float w2, float dx, float dy)
+// ((POI + W0*dx) + W1*dy) + W2, every product and sum rounded to FP32 (the
+// pragma keeps the compiler from fusing a product into the following sum).
+__device__ __forceinline__ float warp_coord(float poi, float w0, float w1, float w2, float dx, float dy)
 {
-    return __fadd_rn(__fadd_rn(__fadd_rn((float)poi, __fmul_rn(w0, dx)), __fmul_rn(w1, dy)), w2);
+#pragma clang fp contract(off)
+    return ((poi + w0 * dx) + w1 * dy) + w2;
 }
 
 // Bicubic value from the materialised LUT cell (16 floats, 4k+l ~ y^k x^l).
@@ -75,15 +77,16 @@ __device__ __forceinline__ float row_dot(uint32_t w, const float wx[4])
 // One unaligned dword per neighbourhood row.  A uniform neighbourhood returns
 // its value exactly (as the LUT does: all higher coefficients vanish), so
 // uniform target regions keep the reference's DarkSubset behaviour.
-__device__ __forceinline__ float interp_fused(const uint8_t* __restrict__ tar, int stride, int ix, int iy,
+// ix, iy >= 1 (interior pixels); offsets are 32-bit, so images are below 4 GiB.
+__device__ __forceinline__ float interp_fused(const uint8_t* __restrict__ tar, uint32_t stride, int ix, int iy,
                                               float tx, float ty)
 {
-    const uint8_t* b = tar + (uint32_t)((iy - 1) * stride + (ix - 1));
+    const uint32_t o0 = __umul24((uint32_t)(iy - 1), stride) + (uint32_t)(ix - 1);
     uint32_t r0, r1, r2, r3;
-    __builtin_memcpy(&r0, b, 4);
-    __builtin_memcpy(&r1, b + stride, 4);
-    __builtin_memcpy(&r2, b + 2 * stride, 4);
-    __builtin_memcpy(&r3, b + 3 * stride, 4);
+    __builtin_memcpy(&r0, tar + o0, 4);
+    __builtin_memcpy(&r1, tar + (o0 + stride), 4);
+    __builtin_memcpy(&r2, tar + (o0 + 2 * stride), 4);
+    __builtin_memcpy(&r3, tar + (o0 + 3 * stride), 4);
     float wx[4], wy[4];
     catmull_rom_w(tx, wx);
     catmull_rom_w(ty, wy);
@@ -91,9 +94,9 @@ __device__ __forceinline__ float interp_fused(const uint8_t* __restrict__ tar, i
     acc = fmaf(wy[1], row_dot(r1, wx), acc);
     acc = fmaf(wy[2], row_dot(r2, wx), acc);
     acc = fmaf(wy[3], row_dot(r3, wx), acc);
-    const uint32_t c = (r1 >> 8) & 255u, rep = c * 0x01010101u;
+    const uint32_t rep = __builtin_amdgcn_perm(r1, r1, 0x01010101u);   // byte 1 of r1, four times
     const bool uniform = (r0 == rep) & (r1 == rep) & (r2 == rep) & (r3 == rep);
-    return uniform ? (float)c : acc;
+    return uniform ? (float)((r1 >> 8) & 255u) : acc;
 }
 
 // W(p) o W(dp)^-1 and p from W: TW_paDIC_ICGN2D_CPU.cpp:713-745 / :875-907,
@@ -154,16 +157,24 @@ __device__ __forceinline__ bool subset_loop(const IcgnParams& P, int px, int py,
 {
     const int Sx = P.Sx, Sy = P.Sy, Wsub = P.Wsub, s = P.s;
     const int sx0 = P.sx0, sy0 = P.sy0, rw = P.rw, rh = P.rh;
+    const uint32_t stride = (uint32_t)P.stride;
+    // lane walks pixels p = lane, lane + 64, ...: column m = p % Wsub, row
+    // l = p / Wsub, kept as the exact float offsets dx = m - Sx, dy = l - Sy
+    // and the 32-bit byte offset into the gradient plane
     const int q64 = 64 / Wsub, r64 = 64 - q64 * Wsub;
-    int l = lane / Wsub, m = lane - (lane / Wsub) * Wsub;
-    const uint32_t* g2 = P.g2 + (size_t)(py - Sy - sy0) * rw + (px - Sx - sx0);
+    const int l0 = lane / Wsub, m0 = lane - l0 * Wsub;
+    float dx = (float)(m0 - Sx), dy = (float)(l0 - Sy);
+    uint32_t go = 4u * (uint32_t)((py - Sy - sy0 + l0) * rw + (px - Sx - sx0 + m0));
+    const float fSx = (float)Sx, fWsub = (float)Wsub, fr64 = (float)r64, fq64 = (float)q64;
+    const uint32_t gstep = 4u * (uint32_t)(q64 * rw + r64), gwrap = 4u * (uint32_t)(rw - Wsub);
+    const uint8_t* g2b = reinterpret_cast<const uint8_t*>(P.g2);
+    const float fpx = (float)px, fpy = (float)py;
     double st = 0.0, stt = 0.0, a0 = 0.0, a1 = 0.0, a2 = 0.0, b0 = 0.0, b1 = 0.0, b2 = 0.0;
     bool out = false;
 #pragma unroll 2
     for (int p = lane; p < s; p += 64) {
-        const float dx = (float)(m - Sx), dy = (float)(l - Sy);
-        const float wx = warp_coord(px, Wm[0], Wm[1], Wm[2], dx, dy);
-        const float wy = warp_coord(py, Wm[3], Wm[4], Wm[5], dx, dy);
+        const float wx = warp_coord(fpx, Wm[0], Wm[1], Wm[2], dx, dy);
+        const float wy = warp_coord(fpy, Wm[3], Wm[4], Wm[5], dx, dy);
         const int ix = (int)wx, iy = (int)wy;          // truncation, as int()
         float T = 0.f;
         bool in = true, interior = true;
@@ -174,28 +185,33 @@ __device__ __forceinline__ bool subset_loop(const IcgnParams& P, int px, int py,
             out |= !in;
         }
         if (INT) {
-            if (in) T = (float)P.tar[(uint32_t)(iy * P.stride + ix)];              // :629
+            if (in) T = (float)P.tar[__umul24((uint32_t)iy, stride) + (uint32_t)ix];     // :629
         } else if (interior) {
-            const float tx = wx - (float)ix, ty = wy - (float)iy;                // :771-772
+            // :771-772; wx, wy >= 1 here, so x - (float)(int)x is fract(x), exactly
+            const float tx = __builtin_amdgcn_fractf(wx), ty = __builtin_amdgcn_fractf(wy);
             if (INTERP != TWDIC_INTERP_FUSED)
                 T = interp_lut(P.lut, (uint32_t)((iy - sy0) * rw + (ix - sx0)), tx, ty);
             else
-                T = interp_fused(P.tar, P.stride, ix, iy, tx, ty);
+                T = interp_fused(P.tar, stride, ix, iy, tx, ty);
         }
-        const uint32_t g = g2[l * rw + m];
+        const uint32_t g = *reinterpret_cast<const uint32_t*>(g2b + go);
         const double gx = (double)(int16_t)(g & 0xffffu), gy = (double)(int16_t)(g >> 16);
-        const double Td = T;
+        const double Td = T, ddx = dx, ddy = dy;
         st += Td;
         stt = fma(Td, Td, stt);
         const double a = gx * Td, b = gy * Td;
         a0 += a;
-        a1 = fma(a, (double)dx, a1);
-        a2 = fma(a, (double)dy, a2);
+        a1 = fma(a, ddx, a1);
+        a2 = fma(a, ddy, a2);
         b0 += b;
-        b1 = fma(b, (double)dx, b1);
-        b2 = fma(b, (double)dy, b2);
-        m += r64; l += q64;
-        if (m >= Wsub) { m -= Wsub; l += 1; }
+        b1 = fma(b, ddx, b1);
+        b2 = fma(b, ddy, b2);
+        // next pixel of this lane: 64 further in row-major subset order
+        dx += fr64;
+        const bool wrap = dx > fSx;
+        dx = wrap ? dx - fWsub : dx;
+        dy += wrap ? fq64 + 1.f : fq64;
+        go += wrap ? gstep + gwrap : gstep;
     }
     acc[0] = st; acc[1] = stt;
     acc[2] = 0.5 * a0; acc[3] = 0.5 * a1; acc[4] = 0.5 * a2;
@@ -216,11 +232,11 @@ template <int INTERP, bool INT>
 __device__ __forceinline__ bool subset_pass(const IcgnParams& P, int px, int py, const float Wm[6], int lane,
                                             double acc[8])
 {
-    const float ex = (float)P.Sx, ey = (float)P.Sy;
-    const float x0 = warp_coord(px, Wm[0], Wm[1], Wm[2], -ex, -ey), x1 = warp_coord(px, Wm[0], Wm[1], Wm[2], ex, -ey);
-    const float x2 = warp_coord(px, Wm[0], Wm[1], Wm[2], -ex, ey), x3 = warp_coord(px, Wm[0], Wm[1], Wm[2], ex, ey);
-    const float y0 = warp_coord(py, Wm[3], Wm[4], Wm[5], -ex, -ey), y1 = warp_coord(py, Wm[3], Wm[4], Wm[5], ex, -ey);
-    const float y2 = warp_coord(py, Wm[3], Wm[4], Wm[5], -ex, ey), y3 = warp_coord(py, Wm[3], Wm[4], Wm[5], ex, ey);
+    const float ex = (float)P.Sx, ey = (float)P.Sy, fx = (float)px, fy = (float)py;
+    const float x0 = warp_coord(fx, Wm[0], Wm[1], Wm[2], -ex, -ey), x1 = warp_coord(fx, Wm[0], Wm[1], Wm[2], ex, -ey);
+    const float x2 = warp_coord(fx, Wm[0], Wm[1], Wm[2], -ex, ey), x3 = warp_coord(fx, Wm[0], Wm[1], Wm[2], ex, ey);
+    const float y0 = warp_coord(fy, Wm[3], Wm[4], Wm[5], -ex, -ey), y1 = warp_coord(fy, Wm[3], Wm[4], Wm[5], ex, -ey);
+    const float y2 = warp_coord(fy, Wm[3], Wm[4], Wm[5], -ex, ey), y3 = warp_coord(fy, Wm[3], Wm[4], Wm[5], ex, ey);
     const bool safe = fminf(fminf(x0, x1), fminf(x2, x3)) >= (float)(P.sx0 + 1) &&
                       fmaxf(fmaxf(x0, x1), fmaxf(x2, x3)) < (float)(P.sx0 + P.rw - 2) &&
                       fminf(fminf(y0, y1), fminf(y2, y3)) >= (float)(P.sy0 + 1) &&

@@ -2,6 +2,7 @@
 // lifetime, image upload, precompute orchestration and launches.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -132,6 +133,86 @@ IcgnParams icgn_params(twdic_ctx* c, const int32_t* poi, int n, float* u, float*
 
 }  // namespace
 
+namespace {
+
+// Everything IC-GN needs before its kernel: reference gradients, the target
+// LUT in the LUT/B-spline modes, and the per-POI reference state, all on the
+// context stream.
+twdic_status icgn_prepare(twdic_ctx* c, const int32_t* d_poi, int n, float* d_u, float* d_v, float* d_p6,
+                          int32_t* d_iters, uint8_t* d_flags, float* d_dpnorm, IcgnParams& p)
+{
+    const twdic_config& g = c->cfg;
+    twdic_status st = ensure_ref_grads(c);
+    if (st != TWDIC_OK) return st;
+    if (g.interp == TWDIC_INTERP_LUT && !c->tar_pre_valid) {
+        // per-frame target LUT (cuBicubicCoefficients, TW_paDIC_cuICGN2D.cu:844-851)
+        TW_HIP(c, launch_bicubic_lut(c->d_tar, c->dstride, g.roi_x, g.roi_y, g.roi_w, g.roi_h, c->d_lut, c->stream));
+        c->tar_pre_valid = true;
+    } else if (g.interp == TWDIC_INTERP_BSPLINE && !c->tar_pre_valid) {
+        // per-frame B-spline LUT (BicubicSplineCoefficients_s, TW_paDIC_ICGN2D_CPU.cpp:333-340)
+        TW_HIP(c, launch_bspline_lut(c->d_tar, c->dstride, g.roi_x, g.roi_y, g.roi_w, g.roi_h, c->d_lut, c->stream));
+        c->tar_pre_valid = true;
+    }
+    if ((size_t)n > c->refstate_cap) {
+        if (c->d_refstate) {
+            TW_HIP(c, hipStreamSynchronize(c->stream));
+            TW_HIP(c, hipStreamSynchronize(c->stream2));
+            TW_HIP(c, hipFree(c->d_refstate));
+            c->d_refstate = nullptr;
+        }
+        TW_HIP(c, hipMalloc((void**)&c->d_refstate, sizeof(double) * kRefStateStride * (size_t)n));
+        c->refstate_cap = (size_t)n;
+        c->refstate_version = ~0ull;
+    }
+    p = icgn_params(c, d_poi, n, d_u, d_v, d_p6, d_iters, d_flags, d_dpnorm);
+    if (c->refstate_version != c->ref_version || c->refstate_poi != d_poi || c->refstate_n != n) {
+        // per-reference state (RefAllSubetsNorm_Kernel + InverseHessian_Kernel,
+        // TW_paDIC_cuICGN2D.cu:883-903), rebuilt when the reference or the POI set changed
+        TW_HIP(c, launch_ref_state(p, c->stream));
+        c->refstate_version = c->ref_version;
+        c->refstate_poi = d_poi;
+        c->refstate_n = n;
+    }
+    return TWDIC_OK;
+}
+
+// The parameter blocks of POIs [base, base + cnt) of a set.
+FftccParams fftcc_chunk(FftccParams p, int base, int cnt)
+{
+    p.poi += 2 * (size_t)base;
+    p.n = cnt;
+    p.u += base; p.v += base; p.zncc += base;
+    if (p.status) p.status += base;
+    return p;
+}
+
+IcgnParams icgn_chunk(IcgnParams p, int base, int cnt)
+{
+    p.poi += 2 * (size_t)base;
+    p.n = cnt;
+    p.refstate += (size_t)base * kRefStateStride;
+    p.u += base; p.v += base;
+    if (p.p6) p.p6 += 6 * (size_t)base;
+    if (p.iters) p.iters += base;
+    if (p.flags) p.flags += base;
+    if (p.dpnorm) p.dpnorm += base;
+    return p;
+}
+
+FftccParams fftcc_params(twdic_ctx* c, const int32_t* d_poi, int n, float* d_u, float* d_v, float* d_zncc,
+                         uint8_t* d_status)
+{
+    FftccParams p{};
+    p.ref = c->d_ref; p.tar = c->d_tar; p.stride = c->dstride;
+    p.img_w = c->cfg.img_w; p.img_h = c->cfg.img_h;
+    p.poi = d_poi; p.n = n; p.N = c->cfg.fft_n; p.tw = c->d_fft_tw;
+    p.tol = 1e-4f;
+    p.u = d_u; p.v = d_v; p.zncc = d_zncc; p.status = d_status;
+    return p;
+}
+
+}  // namespace
+
 extern "C" {
 
 twdic_status twdic_num_poi(const twdic_config* cfg, int32_t* nx, int32_t* ny)
@@ -158,6 +239,8 @@ twdic_status twdic_create(const twdic_config* cfg, int device, twdic_ctx** out)
     c->dstride = cfg->img_w;
     if (const char* v = std::getenv("TWDIC_FFTCC_VARIANT")) c->fftcc_variant = std::atoi(v);
     if (const char* v = std::getenv("TWDIC_ICGN_VARIANT")) c->icgn_variant = std::atoi(v);
+    if (const char* v = std::getenv("TWDIC_FRAME_CHUNKS"))
+        c->frame_chunks = std::max(1, std::min(twdic_ctx::kMaxFrameChunks, std::atoi(v)));
     twdic_status st = TWDIC_OK;
     auto bail = [&](hipError_t e, const char* w) {
         hip_fail(c, e, w);
@@ -166,6 +249,7 @@ twdic_status twdic_create(const twdic_config* cfg, int device, twdic_ctx** out)
     hipError_t e;
     if ((e = hipSetDevice(device)) != hipSuccess) bail(e, "hipSetDevice");
     if (st == TWDIC_OK && (e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking)) != hipSuccess) bail(e, "hipStreamCreate");
+    if (st == TWDIC_OK && (e = hipStreamCreateWithFlags(&c->stream2, hipStreamNonBlocking)) != hipSuccess) bail(e, "hipStreamCreate");
     const size_t img = (size_t)c->dstride * cfg->img_h;
     const size_t roi = (size_t)cfg->roi_w * cfg->roi_h;
     struct { void** p; size_t b; } allocs[] = {
@@ -182,6 +266,8 @@ twdic_status twdic_create(const twdic_config* cfg, int device, twdic_ctx** out)
     }
     for (int i = 0; i < 16 && st == TWDIC_OK; ++i)
         if ((e = hipEventCreate(&c->ev[i])) != hipSuccess) bail(e, "hipEventCreate");
+    for (int i = 0; i <= twdic_ctx::kMaxFrameChunks && st == TWDIC_OK; ++i)
+        if ((e = hipEventCreateWithFlags(&c->pev[i], hipEventDisableTiming)) != hipSuccess) bail(e, "hipEventCreate");
     if (st == TWDIC_OK) {
         // twiddles exp(-2 pi i e / N), rounded from double
         float2 tw[kMaxFftN];
@@ -206,6 +292,7 @@ twdic_status twdic_destroy(twdic_ctx* c)
     if (!c) return TWDIC_EINVAL;
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
+    if (c->stream2) (void)hipStreamSynchronize(c->stream2);
     void* bufs[] = {c->d_ref, c->d_tar, c->d_rx, c->d_ry, c->d_tx, c->d_ty, c->d_txy, c->d_lut,
                     c->d_fft_tw, c->d_refstate, c->d_g2};
     for (void* b : bufs)
@@ -213,6 +300,9 @@ twdic_status twdic_destroy(twdic_ctx* c)
     if (c->h_stage) (void)hipHostFree(c->h_stage);
     for (auto& ev : c->ev)
         if (ev) (void)hipEventDestroy(ev);
+    for (auto& ev : c->pev)
+        if (ev) (void)hipEventDestroy(ev);
+    if (c->stream2) (void)hipStreamDestroy(c->stream2);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return TWDIC_OK;
@@ -269,13 +359,7 @@ twdic_status twdic_fftcc(twdic_ctx* c, const int32_t* d_poi, int n, float* d_u, 
 {
     if (!c || n < 0 || (n > 0 && (!d_poi || !d_u || !d_v || !d_zncc))) return fail(c, TWDIC_EINVAL, "twdic_fftcc: bad arguments");
     if (!c->ref_loaded || !c->tar_loaded) return fail(c, TWDIC_EINVAL, "twdic_fftcc: images not set");
-    FftccParams p{};
-    p.ref = c->d_ref; p.tar = c->d_tar; p.stride = c->dstride;
-    p.img_w = c->cfg.img_w; p.img_h = c->cfg.img_h;
-    p.poi = d_poi; p.n = n; p.N = c->cfg.fft_n; p.tw = c->d_fft_tw;
-    p.tol = 1e-4f;
-    p.u = d_u; p.v = d_v; p.zncc = d_zncc; p.status = d_status;
-    TW_HIP(c, launch_fftcc(p, c->fftcc_variant, c->stream));
+    TW_HIP(c, launch_fftcc(fftcc_params(c, d_poi, n, d_u, d_v, d_zncc, d_status), c->fftcc_variant, c->stream));
     return TWDIC_OK;
 }
 
@@ -302,38 +386,10 @@ twdic_status twdic_icgn(twdic_ctx* c, const int32_t* d_poi, int n, float* d_u, f
     if (!c || n < 0 || (n > 0 && (!d_poi || !d_u || !d_v))) return fail(c, TWDIC_EINVAL, "twdic_icgn: bad arguments");
     if (!c->ref_loaded || !c->tar_loaded) return fail(c, TWDIC_EINVAL, "twdic_icgn: images not set");
     if (n == 0) return TWDIC_OK;
-    const twdic_config& g = c->cfg;
-    twdic_status st = ensure_ref_grads(c);
+    IcgnParams p;
+    twdic_status st = icgn_prepare(c, d_poi, n, d_u, d_v, d_p6, d_iters, d_flags, d_dpnorm, p);
     if (st != TWDIC_OK) return st;
-    if (g.interp == TWDIC_INTERP_LUT && !c->tar_pre_valid) {
-        // per-frame target LUT (cuBicubicCoefficients, TW_paDIC_cuICGN2D.cu:844-851)
-        TW_HIP(c, launch_bicubic_lut(c->d_tar, c->dstride, g.roi_x, g.roi_y, g.roi_w, g.roi_h, c->d_lut, c->stream));
-        c->tar_pre_valid = true;
-    } else if (g.interp == TWDIC_INTERP_BSPLINE && !c->tar_pre_valid) {
-        // per-frame B-spline LUT (BicubicSplineCoefficients_s, TW_paDIC_ICGN2D_CPU.cpp:333-340)
-        TW_HIP(c, launch_bspline_lut(c->d_tar, c->dstride, g.roi_x, g.roi_y, g.roi_w, g.roi_h, c->d_lut, c->stream));
-        c->tar_pre_valid = true;
-    }
-    if ((size_t)n > c->refstate_cap) {
-        if (c->d_refstate) {
-            TW_HIP(c, hipStreamSynchronize(c->stream));
-            TW_HIP(c, hipFree(c->d_refstate));
-            c->d_refstate = nullptr;
-        }
-        TW_HIP(c, hipMalloc((void**)&c->d_refstate, sizeof(double) * kRefStateStride * (size_t)n));
-        c->refstate_cap = (size_t)n;
-        c->refstate_version = ~0ull;
-    }
-    IcgnParams p = icgn_params(c, d_poi, n, d_u, d_v, d_p6, d_iters, d_flags, d_dpnorm);
-    if (c->refstate_version != c->ref_version || c->refstate_poi != d_poi || c->refstate_n != n) {
-        // per-reference state (RefAllSubetsNorm_Kernel + InverseHessian_Kernel,
-        // TW_paDIC_cuICGN2D.cu:883-903), rebuilt when the reference or the POI set changed
-        TW_HIP(c, launch_ref_state(p, c->stream));
-        c->refstate_version = c->ref_version;
-        c->refstate_poi = d_poi;
-        c->refstate_n = n;
-    }
-    TW_HIP(c, launch_icgn(p, g.interp, c->icgn_variant, c->stream));
+    TW_HIP(c, launch_icgn(p, c->cfg.interp, c->icgn_variant, c->stream));
     return TWDIC_OK;
 }
 
@@ -347,9 +403,35 @@ twdic_status twdic_reset_poi_state(twdic_ctx* c)
 twdic_status twdic_frame(twdic_ctx* c, const int32_t* d_poi, int n, float* d_u, float* d_v, float* d_zncc,
                          float* d_p6, int32_t* d_iters, uint8_t* d_flags)
 {
-    twdic_status st = twdic_fftcc(c, d_poi, n, d_u, d_v, d_zncc, nullptr);
+    if (!c || n < 0 || (n > 0 && (!d_poi || !d_u || !d_v || !d_zncc))) return fail(c, TWDIC_EINVAL, "twdic_frame: bad arguments");
+    if (!c->ref_loaded || !c->tar_loaded) return fail(c, TWDIC_EINVAL, "twdic_frame: images not set");
+    if (n == 0) return TWDIC_OK;
+    const int chunks = n >= 4096 * c->frame_chunks ? c->frame_chunks : 1;
+    if (chunks == 1) {
+        twdic_status st = twdic_fftcc(c, d_poi, n, d_u, d_v, d_zncc, nullptr);
+        if (st != TWDIC_OK) return st;
+        return twdic_icgn(c, d_poi, n, d_u, d_v, d_p6, d_iters, d_flags, nullptr);
+    }
+    // the per-reference state of the whole set first (context stream), then
+    // chunk i: FFT-CC on the context stream, IC-GN on the second stream once
+    // that chunk's integer guesses exist; FFT-CC of chunk i + 1 overlaps it.
+    // Results are those of the two calls back to back: the same kernels over
+    // the same POIs, each POI's work independent of the others.
+    IcgnParams ip;
+    twdic_status st = icgn_prepare(c, d_poi, n, d_u, d_v, d_p6, d_iters, d_flags, nullptr, ip);
     if (st != TWDIC_OK) return st;
-    return twdic_icgn(c, d_poi, n, d_u, d_v, d_p6, d_iters, d_flags, nullptr);
+    const FftccParams fp = fftcc_params(c, d_poi, n, d_u, d_v, d_zncc, nullptr);
+    for (int i = 0; i < chunks; ++i) {
+        const int b0 = (int)((long long)n * i / chunks), b1 = (int)((long long)n * (i + 1) / chunks);
+        TW_HIP(c, launch_fftcc(fftcc_chunk(fp, b0, b1 - b0), c->fftcc_variant, c->stream));
+        TW_HIP(c, hipEventRecord(c->pev[i], c->stream));
+        TW_HIP(c, hipStreamWaitEvent(c->stream2, c->pev[i], 0));
+        TW_HIP(c, launch_icgn(icgn_chunk(ip, b0, b1 - b0), c->cfg.interp, c->icgn_variant, c->stream2));
+    }
+    // later work on the context stream (the next target, a sync) waits for IC-GN
+    TW_HIP(c, hipEventRecord(c->pev[chunks], c->stream2));
+    TW_HIP(c, hipStreamWaitEvent(c->stream, c->pev[chunks], 0));
+    return TWDIC_OK;
 }
 
 twdic_status twdic_get_buffer(twdic_ctx* c, int which, void** d_ptr, size_t* bytes)

@@ -170,5 +170,11 @@ struct twdic_ctx {
     uint8_t* h_stage = nullptr;       // pinned staging for host images
     size_t h_stage_bytes = 0;
     hipEvent_t ev[16] = {};
+    // frame pipeline: FFT-CC of POI chunk i + 1 (on `stream`) overlaps IC-GN of
+    // chunk i (on `stream2`)
+    static constexpr int kMaxFrameChunks = 16;
+    hipStream_t stream2 = nullptr;
+    hipEvent_t pev[kMaxFrameChunks + 1] = {};
+    int frame_chunks = 1;             // TWDIC_FRAME_CHUNKS; 1 runs the two stages back to back
     std::string err;
 };
