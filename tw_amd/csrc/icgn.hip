@@ -162,7 +162,11 @@ __device__ __forceinline__ bool subset_loop(const IcgnParams& P, int px, int py,
     // l = p / Wsub, kept as the exact float offsets dx = m - Sx, dy = l - Sy
     // and the 32-bit byte offset into the gradient plane
     const int q64 = 64 / Wsub, r64 = 64 - q64 * Wsub;
-    const int l0 = lane / Wsub, m0 = lane - l0 * Wsub;
+    // recomputed per pass: kept live across the iteration loop, this start
+    // state is what the register allocator spills
+    int lane_v = lane;
+    asm volatile("" : "+v"(lane_v));
+    const int l0 = lane_v / Wsub, m0 = lane_v - l0 * Wsub;
     float dx = (float)(m0 - Sx), dy = (float)(l0 - Sy);
     uint32_t go = 4u * (uint32_t)((py - Sy - sy0 + l0) * rw + (px - Sx - sx0 + m0));
     const float fSx = (float)Sx, fWsub = (float)Wsub, fr64 = (float)r64, fq64 = (float)q64;
@@ -171,7 +175,7 @@ __device__ __forceinline__ bool subset_loop(const IcgnParams& P, int px, int py,
     const float fpx = (float)px, fpy = (float)py;
     double st = 0.0, stt = 0.0, a0 = 0.0, a1 = 0.0, a2 = 0.0, b0 = 0.0, b1 = 0.0, b2 = 0.0;
     bool out = false;
-#pragma unroll 2
+#pragma unroll 1
     for (int p = lane; p < s; p += 64) {
         const float wx = warp_coord(fpx, Wm[0], Wm[1], Wm[2], dx, dy);
         const float wy = warp_coord(fpy, Wm[3], Wm[4], Wm[5], dx, dy);
