@@ -146,6 +146,87 @@ def serving(a):
     print(json.dumps(out))
 
 
+def batch(a):
+    """The batched configuration: 64 speckle pairs of sizes drawn from
+    {512, 1024, 2048, 4096}^2 (0.015 spots/px^2), each with a random affine
+    field (strains within +-1 %, translations within +-8 px), subset radius
+    from {10, 15, 20, 30} (FFT-CC window 32, 32, 64, 64) and grid step from
+    {4, 5, 10}; POIs on the grid from a border of max(N/2, S + 2 +
+    ceil(8 + size/100)) px (the largest displacement the field reaches).  One
+    step is FFT-CC then IC-GN over every POI of every pair, through the
+    batched API (direct form: per pair, the single-context kernels)."""
+    from tw_amd import api
+    rng = np.random.default_rng(1003)
+    t_gen = time.perf_counter()
+    alloc = api.Context(api.make_config(64, 64, 1, 1, 62, 62, 2, 2, 4, 4, 0, 0, 8))
+    pairs, keep, n_tot, classes = [], [], 0, {}
+    for i in range(a.pairs):
+        size = int(rng.choice([512, 1024, 2048, 4096]))
+        S = int(rng.choice([10, 15, 20, 30]))
+        g = int(rng.choice([4, 5, 10]))
+        N = 32 if S <= 15 else 64
+        d = api.affine(*rng.uniform(-8, 8, 2), *rng.uniform(-0.01, 0.01, 3), size / 2, size / 2)
+        ref, tar = api.synth_pair(size, size, int(0.015 * size * size), 1003000 + i, d, noise_sigma=2.0)
+        m = max(N // 2, S + 2 + int(np.ceil(8 + 0.01 * size)))
+        ys = np.arange(m, size - m, g)
+        yy, xx = np.meshgrid(ys, ys, indexing="ij")
+        poi = np.stack([yy.ravel(), xx.ravel()], 1).astype(np.int32)
+        n = len(poi)
+        b = dict(ref=alloc.array(ref.shape, np.uint8).upload(ref), tar=alloc.array(tar.shape, np.uint8).upload(tar),
+                 poi=alloc.array(poi.shape, np.int32).upload(poi), u=alloc.array(n, np.float32),
+                 v=alloc.array(n, np.float32), p6=alloc.array((n, 6), np.float32), it=alloc.array(n, np.int32),
+                 fl=alloc.array(n, np.uint8))
+        keep.append(b)
+        pairs.append(api.Pair(b["ref"].ptr, b["tar"].ptr, size, size, size, 1, 1, size - 2, size - 2, S, S, N, 20,
+                              1e-3, b["poi"].ptr, n, b["u"].ptr, b["v"].ptr, None, b["p6"].ptr, b["it"].ptr,
+                              b["fl"].ptr))
+        n_tot += n
+        classes[f"{size}^2 S{S} g{g}"] = classes.get(f"{size}^2 S{S} g{g}", 0) + 1
+        if i % 8 == 7:
+            print(f"batch: {i + 1} pairs synthesised, {n_tot} POIs", file=sys.stderr, flush=True)
+    alloc.sync()
+    t_gen = time.perf_counter() - t_gen
+    with api.Batch() as bt:
+        bt.set_pairs(pairs)
+        bt.run()                                   # reference state + first frame
+        bt.sync()
+        for _ in range(a.warmup_batch):
+            bt.run()
+        bt.sync()
+        bt.record(0)
+        for _ in range(a.steps_batch):
+            bt.run()
+        bt.record(1)
+        bt.sync()
+        ms = bt.elapsed_ms(0, 1) / a.steps_batch
+        lat = []
+        for _ in range(a.steps_batch):
+            bt.record(2)
+            bt.run()
+            bt.record(3)
+            bt.sync()
+            lat.append(bt.elapsed_ms(2, 3))
+        bt.record(4)
+        bt.run_icgn()
+        bt.record(5)
+        bt.sync()
+        icgn_ms = bt.elapsed_ms(4, 5)
+    fl = np.concatenate([b["fl"].download() for b in keep])
+    out = {
+        "metric": "batched form: POIs/s over 64 mixed pairs (FFT-CC + IC-GN)",
+        "value": n_tot / (ms * 1e-3), "unit": "POIs/s", "n_gpus": 1, "steps": a.steps_batch,
+        "warmup": a.warmup_batch, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "fp32", "data": "synthetic",
+        "config": {"workload": "batched64_mixed", "pairs": a.pairs, "pois": n_tot, "classes": classes,
+                   "execution": "direct form (per-pair single-context kernels)"},
+        "icgn_pois_per_s": n_tot / (icgn_ms * 1e-3),
+        "latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99))},
+        "flags": np.bincount(fl, minlength=5).tolist(), "synth_s": t_gen,
+    }
+    alloc.close()
+    print(json.dumps(out))
+
+
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -156,9 +237,12 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sample", type=int, default=0,
                     help="POIs in the CPU baseline sample (0: the whole frame, about 15 s on 16 cores)")
-    ap.add_argument("--workload", choices=["frame", "serving"], default="frame",
+    ap.add_argument("--workload", choices=["frame", "serving", "batch"], default="frame",
                     help="frame: the 2048^2 plain frame step (default); serving: Poisson request stream "
-                         "through the serving loop")
+                         "through the serving loop; batch: 64 mixed pairs through the batched API")
+    ap.add_argument("--pairs", type=int, default=64, help="batch: image pairs")
+    ap.add_argument("--steps-batch", type=int, default=5, help="batch: timed steps")
+    ap.add_argument("--warmup-batch", type=int, default=2, help="batch: untimed steps")
     ap.add_argument("--rate", type=float, default=2000.0, help="serving: mean arrivals per second")
     ap.add_argument("--duration", type=float, default=10.0, help="serving: seconds of arrivals")
     ap.add_argument("--slots", type=int, default=8, help="serving: requests in flight at once")
@@ -200,11 +284,11 @@ def main():
     if a.copy_calibration:
         copy_calibration(a.copy_calibration, 4)
         return
-    if a.workload == "serving":
+    if a.workload in ("serving", "batch"):
         if a.gpus != 1:
             print("bench.py runs on one GPU only", file=sys.stderr)
             sys.exit(2)
-        serving(a)
+        serving(a) if a.workload == "serving" else batch(a)
         return
     if a.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {a.gpus}); the multi-GPU layout is a design "

@@ -1,9 +1,15 @@
 /*
  * twdic_batch.h -- batched form of the paDIC hot path: many image pairs of
- * uneven sizes, mixed subset radii and FFT-CC windows processed by persistent
- * kernels that drain device work queues, one launch per stage and window
- * class, replayable as a hipGraph, with launch parameters chosen by an
- * autotuner.
+ * uneven sizes, mixed subset radii and FFT-CC windows behind one handle, with
+ * resident per-pair reference state.
+ *
+ * Two executions.  The default build runs the direct form: per pair, the
+ * single-context kernels in pair order on the batch stream.  The persistent
+ * form (kernels that drain device work queues, one launch per stage and window
+ * class, hipGraph replay, launch parameters chosen by an autotuner) is compiled
+ * only with -DTWDIC_BATCH_EXEC: its first run on an MI355X stalled and the
+ * cause has not been found.  In the default build the tuning calls are
+ * accepted and have no effect, and twdic_batch_autotune returns TWDIC_EINVAL.
  *
  * No counterpart in the reference, which runs one image pair per
  * cuFFTCC2D/cuICGN2D object and one CUDA block per POI
@@ -24,8 +30,9 @@ typedef struct twdic_batch twdic_batch;
 
 /* One image pair.  Images, POIs and outputs are device buffers owned by the
  * caller (borrowed for the lifetime of the batch).  Subsets are
- * (2 sy + 1) x (2 sx + 1); fft_n = 32 or 64 uses the radix-4 engine, other even
- * values up to 64 the direct-DFT engine.  Any output may be NULL except
+ * (2 sy + 1) x (2 sx + 1); fft_n is any even value up to 64 (the direct form
+ * runs every window size on the int8 matrix cores, as the single-context API
+ * does).  Any output may be NULL except
  * d_u/d_v.  IC-GN interpolation is the fused bicubic evaluation. */
 typedef struct {
     const uint8_t* d_ref;
@@ -78,9 +85,9 @@ twdic_status twdic_batch_set_targets(twdic_batch* b, const uint8_t* const* d_tar
 twdic_status twdic_batch_set_references(twdic_batch* b, const uint8_t* const* d_refs);
 
 /* One batched frame step, asynchronous on the batch stream: reference
- * gradients and per-POI state when stale, then FFT-CC over every POI (one
- * persistent launch per window class), then IC-GN over every POI from the
- * FFT-CC guesses (one persistent launch). */
+ * gradients and per-POI state when stale, then FFT-CC over every POI, then
+ * IC-GN over every POI from the FFT-CC guesses (direct form: one launch per
+ * pair and stage; persistent form: one launch per window class and stage). */
 twdic_status twdic_batch_run(twdic_batch* b);
 /* IC-GN only, from the guesses already in d_u/d_v. */
 twdic_status twdic_batch_run_icgn(twdic_batch* b);

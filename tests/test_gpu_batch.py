@@ -18,6 +18,7 @@ SPECS = [  # (w, h, S, grid, N, deformation seed)
     (512, 512, 20, 11, 40, 3),
     (300, 200, 12, 6, 24, 4),
     (448, 448, 20, 13, 64, 5),
+    (512, 480, 30, 14, 64, 6),
 ]
 
 
@@ -151,7 +152,7 @@ def test_autotune_table_roundtrip(data, tmp_path):
         bt.autotune(str(table), reps=1)
         chosen = bt.tuning
         lines = table.read_text().strip().splitlines()
-        assert len(lines) == 1 and "pairs=5" in lines[0]
+        assert len(lines) == 1 and f"pairs={len(SPECS)}" in lines[0]
         bt.run()
         bt.sync()
         first = [bufs.results(i) for i in range(len(data))]

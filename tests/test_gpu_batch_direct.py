@@ -1,0 +1,74 @@
+"""Batched form, direct execution (the default build): the pairs of one batch
+run the single-context kernels pair by pair, so every per-POI result must be
+bit-identical to a single context on the same pair, alone or in any mix,
+across new targets, new references and repeated runs.  The pairs mix image
+sizes, subset radii and FFT-CC windows (32, 64 and the reference's own 2S
+sizes 24 and 40)."""
+import numpy as np
+import pytest
+
+from tests.test_gpu_batch import SPECS, _Buffers, _make, _same, _single
+from tw_amd import api
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def data():
+    return [_make(s) for s in SPECS]
+
+
+def test_direct_batch_matches_single_context_bitwise(data):
+    singles = [_single(p) for p in data]
+    bufs = _Buffers(data)
+    with api.Batch() as bt:
+        bt.set_pairs(bufs.pairs(data))
+        info = bt.info()
+        assert info.total_pois == sum(len(p["poi"]) for p in data)
+        for rep in range(2):                       # the second run reuses the resident state
+            bt.run()
+            bt.sync()
+            for i in range(len(data)):
+                _same(bufs.results(i), singles[i])
+        bt.run_icgn()                              # IC-GN again from the refined guesses
+        bt.sync()
+    assert np.mean(np.concatenate([s["flags"] for s in singles]) == 0) > 0.95
+    bufs.alloc.close()
+
+
+def test_direct_alone_equals_in_mix_new_targets_and_references(data):
+    bufs = _Buffers(data)
+    with api.Batch() as bt:
+        bt.set_pairs(bufs.pairs(data, [2]))
+        bt.run()
+        bt.sync()
+        alone = bufs.results(2)
+        bt.set_pairs(bufs.pairs(data))
+        bt.run()
+        bt.sync()
+        _same(bufs.results(2), alone)
+        # a new target frame for pair 0 (reference state kept resident)
+        w, h, S, g, N, seed = SPECS[0]
+        _, t2 = api.synth_pair(w, h, int(0.015 * w * h), 500 + seed, api.affine(-2.5, 1.25, 0.004, 0, 0, w / 2, h / 2),
+                               want_ref=False)
+        dt2 = bufs.alloc.array((h, w), np.uint8).upload(t2)
+        bt.set_targets([dt2.ptr] + [None] * (len(data) - 1))
+        bt.run()
+        bt.sync()
+        _same(bufs.results(0), _single(dict(data[0], tar=t2)))
+        # a new reference for pair 1: its per-POI state is rebuilt
+        p1 = data[1]
+        r2 = np.ascontiguousarray(np.roll(p1["ref"], (1, -2), (0, 1)))
+        dr2 = bufs.alloc.array(r2.shape, np.uint8).upload(r2)
+        bufs.alloc.sync()
+        bt.set_references([None, dr2.ptr] + [None] * (len(data) - 2))
+        bt.run()
+        bt.sync()
+        _same(bufs.results(1), _single(dict(p1, ref=r2)))
+    bufs.alloc.close()
+
+
+def test_persistent_form_calls_are_refused():
+    with api.Batch() as bt:
+        with pytest.raises(api.TwdicError):
+            bt.autotune(None, reps=1)

@@ -64,14 +64,18 @@ twdic_status bfail(twdic_batch* b, twdic_status st, const std::string& what)
 
 int fclass(int N) { return N == 64 ? 0 : N == 32 ? 1 : 2; }
 
-// Device execution of the batched form is switched off: its first run on an
-// MI355X stalled without output and the cause has not been found.  Building
-// with -DTWDIC_BATCH_EXEC switches it back on.
+// The persistent-kernel form (work queues, hipGraph replay, autotuner)
+// stalled on its first run on an MI355X and the cause has not been found; it
+// is compiled only with -DTWDIC_BATCH_EXEC.  Without it, a batch runs in the
+// direct form: per pair, the single-context kernels (launch_fftcc,
+// launch_icgn, launch_ref_state) in pair order on the batch stream.
 #ifdef TWDIC_BATCH_EXEC
 #define TB_EXEC_GATE(b) (void)0
+constexpr bool kPersistent = true;
 #else
 #define TB_EXEC_GATE(b) \
-    return bfail((b), TWDIC_EINVAL, "batched execution is switched off in this build (stalled on its first run)")
+    return bfail((b), TWDIC_EINVAL, "the persistent batched form is switched off in this build (stalled on its first run)")
+constexpr bool kPersistent = false;
 #endif
 
 void release(twdic_batch* b)
@@ -145,6 +149,34 @@ twdic_status enqueue_frame(twdic_batch* b, bool fftcc, bool icgn)
     if (icgn)
         TB_HIP(b, launch_batch_icgn(b->d_desc, b->d_items, (int)b->total, b->d_counters + CNT_ICGN,
                                     std::max(1, b->tune.icgn_chunk), persistent_blocks_icgn(b->device), b->stream));
+    return TWDIC_OK;
+}
+
+// Direct form: per-pair launches of the single-context kernels.
+twdic_status build_state_direct(twdic_batch* b)
+{
+    for (size_t i = 0; i < b->pairs.size(); ++i) {
+        const IcgnParams& q = b->h_desc[i].icgn;
+        TB_HIP(b, launch_gradients(q.ref, q.stride, q.sx0, q.sy0, q.rw, q.rh, nullptr, nullptr, nullptr,
+                                   const_cast<uint32_t*>(q.g2), b->stream));
+        if (q.n > 0) TB_HIP(b, launch_ref_state(q, b->stream));
+    }
+    b->state_valid = true;
+    return TWDIC_OK;
+}
+
+twdic_status run_direct(twdic_batch* b, bool fftcc, bool icgn)
+{
+    if (!b->state_valid) {
+        twdic_status st = build_state_direct(b);
+        if (st != TWDIC_OK) return st;
+    }
+    if (fftcc)
+        for (const auto& d : b->h_desc)
+            if (d.fftcc.n > 0) TB_HIP(b, launch_fftcc(d.fftcc, kFftccMma, b->stream));
+    if (icgn)
+        for (const auto& d : b->h_desc)
+            if (d.icgn.n > 0) TB_HIP(b, launch_icgn(d.icgn, TWDIC_INTERP_FUSED, 5, b->stream));
     return TWDIC_OK;
 }
 
@@ -309,9 +341,9 @@ twdic_status twdic_batch_set_references(twdic_batch* b, const uint8_t* const* d_
 twdic_status twdic_batch_run(twdic_batch* b)
 {
     if (!b) return TWDIC_EINVAL;
-    TB_EXEC_GATE(b);
     if (!b->total) return TWDIC_OK;
     (void)hipSetDevice(b->device);
+    if (!kPersistent) return run_direct(b, true, true);
     if (!b->state_valid) {
         twdic_status st = build_state(b);
         if (st != TWDIC_OK) return st;
@@ -342,8 +374,9 @@ twdic_status twdic_batch_run(twdic_batch* b)
 twdic_status twdic_batch_run_icgn(twdic_batch* b)
 {
     if (!b) return TWDIC_EINVAL;
-    TB_EXEC_GATE(b);
     if (!b->total) return TWDIC_OK;
+    (void)hipSetDevice(b->device);
+    if (!kPersistent) return run_direct(b, false, true);
     if (!b->state_valid) {
         twdic_status st = build_state(b);
         if (st != TWDIC_OK) return st;
