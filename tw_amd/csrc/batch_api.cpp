@@ -6,6 +6,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <fstream>
 #include <map>
@@ -46,6 +47,12 @@ struct twdic_batch {
     bool graph_dirty = true;
     hipEvent_t ev[16] = {};
     size_t bytes = 0;
+    // direct form: pairs are issued round-robin on these streams, which fork
+    // from and join back into `stream`
+    static constexpr int kLanes = 4;
+    hipStream_t lane[kLanes] = {};
+    hipEvent_t fork = nullptr, join[kLanes] = {};
+    int lanes = 4;                    // TWDIC_BATCH_STREAMS (1..4)
 };
 
 namespace {
@@ -171,12 +178,22 @@ twdic_status run_direct(twdic_batch* b, bool fftcc, bool icgn)
         twdic_status st = build_state_direct(b);
         if (st != TWDIC_OK) return st;
     }
-    if (fftcc)
-        for (const auto& d : b->h_desc)
-            if (d.fftcc.n > 0) TB_HIP(b, launch_fftcc(d.fftcc, kFftccMma, b->stream));
-    if (icgn)
-        for (const auto& d : b->h_desc)
-            if (d.icgn.n > 0) TB_HIP(b, launch_icgn(d.icgn, TWDIC_INTERP_FUSED, 5, b->stream));
+    // fork: every lane stream waits for the work already on the batch stream
+    const int L = std::max(1, std::min(b->lanes, twdic_batch::kLanes));
+    TB_HIP(b, hipEventRecord(b->fork, b->stream));
+    for (int k = 0; k < L; ++k) TB_HIP(b, hipStreamWaitEvent(b->lane[k], b->fork, 0));
+    for (size_t i = 0; i < b->h_desc.size(); ++i) {
+        const PairDesc& d = b->h_desc[i];
+        hipStream_t s = b->lane[i % L];
+        if (fftcc && d.fftcc.n > 0) TB_HIP(b, launch_fftcc(d.fftcc, kFftccMma, s));
+        if (icgn && d.icgn.n > 0) TB_HIP(b, launch_icgn(d.icgn, TWDIC_INTERP_FUSED, 5, s));
+    }
+    // join: later work on the batch stream (events, syncs, the next step)
+    // waits for every lane
+    for (int k = 0; k < L; ++k) {
+        TB_HIP(b, hipEventRecord(b->join[k], b->lane[k]));
+        TB_HIP(b, hipStreamWaitEvent(b->stream, b->join[k], 0));
+    }
     return TWDIC_OK;
 }
 
@@ -214,6 +231,15 @@ twdic_status twdic_batch_create(int device, twdic_batch** out)
             twdic_batch_destroy(b);
             return TWDIC_EHIP;
         }
+    bool ok = hipEventCreateWithFlags(&b->fork, hipEventDisableTiming) == hipSuccess;
+    for (int k = 0; k < twdic_batch::kLanes && ok; ++k)
+        ok = hipStreamCreateWithFlags(&b->lane[k], hipStreamNonBlocking) == hipSuccess &&
+             hipEventCreateWithFlags(&b->join[k], hipEventDisableTiming) == hipSuccess;
+    if (!ok) {
+        twdic_batch_destroy(b);
+        return TWDIC_EHIP;
+    }
+    if (const char* v = std::getenv("TWDIC_BATCH_STREAMS")) b->lanes = std::atoi(v);
     *out = b;
     return TWDIC_OK;
 }
@@ -222,9 +248,16 @@ twdic_status twdic_batch_destroy(twdic_batch* b)
 {
     if (!b) return TWDIC_EINVAL;
     (void)hipSetDevice(b->device);
+    for (auto& l : b->lane)
+        if (l) (void)hipStreamSynchronize(l);
     release(b);
     for (auto& e : b->ev)
         if (e) (void)hipEventDestroy(e);
+    if (b->fork) (void)hipEventDestroy(b->fork);
+    for (int k = 0; k < twdic_batch::kLanes; ++k) {
+        if (b->join[k]) (void)hipEventDestroy(b->join[k]);
+        if (b->lane[k]) (void)hipStreamDestroy(b->lane[k]);
+    }
     if (b->stream) (void)hipStreamDestroy(b->stream);
     delete b;
     return TWDIC_OK;
