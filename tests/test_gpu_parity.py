@@ -205,3 +205,41 @@ def test_frame_end_to_end(c512):
     o = oracle.icgn(c512.ref, c512.tar, c512.roi, 15, 15, poi, ex["u"].astype(np.float32),
                     ex["v"].astype(np.float32))
     check_icgn(g, o, gtol=3e-6)
+
+
+@pytest.mark.parametrize("seed", [0, 1, 6])
+def test_icgn_flat_target_reached_in_a_bicubic_pass(seed):
+    """A textured reference against a target with a flat block: from guesses
+    near the block edge, IC-GN walks some subsets fully into the block, where
+    the bicubic pass sees zero spread and the CPU path flags DarkSubset
+    (TW_paDIC_ICGN2D_CPU.cpp:651-653) at iteration 2 or later.  Every such POI
+    must get the same flag and iteration count (the uniform-neighbourhood value
+    has to be exact for this).  Elsewhere near-flat subsets make IC-GN chaotic:
+    where the FP32 and FP64 CPU paths disagree on the flag (seed 6 has one such
+    POI), the GPU's flag must be one of the two; everywhere else it must equal
+    the FP32 path's."""
+    rng = np.random.default_rng(seed)
+    ref, _ = api.synth_pair(192, 192, 900, 700 + seed, noise_sigma=0.0, want_tar=False)
+    tar = ref.copy()
+    y0, x0 = rng.integers(40, 100, 2)
+    tar[y0:y0 + 60, x0:x0 + 60] = 90
+    S = 6
+    ys, xs = np.meshgrid(np.arange(y0 - 8, y0 + 68, 2), np.arange(x0 - 8, x0 + 68, 2), indexing="ij")
+    poi = np.stack([ys.ravel(), xs.ravel()], 1).astype(np.int32)
+    poi = poi[(poi[:, 0] > 20) & (poi[:, 0] < 172) & (poi[:, 1] > 20) & (poi[:, 1] < 172)]
+    u0 = rng.integers(-3, 4, len(poi)).astype(np.float32)
+    v0 = rng.integers(-3, 4, len(poi)).astype(np.float32)
+    case = cases.Case("flat", 192, S, 2, 16, (1, 1, 190, 190), 0, ref, tar, None)
+    o = oracle.icgn(ref, tar, case.roi, S, S, poi, u0, v0)
+    o64 = oracle.icgn(ref, tar, case.roi, S, S, poi, u0.astype(np.float64), v0.astype(np.float64), precision="f64")
+    late = (o["flags"] == oracle.DARK_SUBSET) & (o["iters"] >= 2)
+    assert late.sum() >= 2
+    with ctx_for(case) as ctx:
+        g = run_icgn(ctx, poi, u0, v0)
+    assert np.array_equal(g["flags"][late], o["flags"][late])
+    assert np.array_equal(g["iters"][late], o["iters"][late])
+    stable = o["flags"] == o64["flags"]
+    assert stable.mean() > 0.99
+    assert np.array_equal(g["flags"][stable], o["flags"][stable]), np.nonzero(stable & (g["flags"] != o["flags"]))
+    either = (g["flags"] == o["flags"]) | (g["flags"] == o64["flags"])
+    assert np.all(either), np.nonzero(~either)
