@@ -391,7 +391,8 @@ def write_csv(path, poi_yx, int_u, int_v, p6, iters, zncc=None, fftcc_status=Non
 
 
 class Batch:
-    """A twdic_batch: persistent-kernel processing of many image pairs.  Device
+    """A twdic_batch: many image pairs behind one handle (the direct form runs the
+    single-context kernels per pair; the persistent form is off in this build).  Device
     buffers are allocated through a helper Context-free allocator owned by the
     batch object (the caller's arrays are borrowed)."""
 
