@@ -10,7 +10,8 @@
 //     reference's cuFFT path can pick a different shift on near ties);
 //   * errors throw std::runtime_error instead of exiting the process;
 //   * the FFT-CC window edge defaults to 2 * iSubsetX as in the reference and
-//     can be changed with setFFTWindow (32 or 64 use the radix-4 engine).
+//     can be changed with setFFTWindow (any even N up to 64; every size runs
+//     on the int8 matrix cores).
 #ifndef TWAMD_TW_PADIC_CUFFTCC2D_H
 #define TWAMD_TW_PADIC_CUFFTCC2D_H
 
