@@ -34,8 +34,7 @@ HIP_FLAGS = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     f"-I{INCLUDE}", f"-I{CSRC}",
 ]
-CXX_FLAGS = ["-O3", "-std=c++17", "-fPIC", "-Wall", "-fopenmp", f"-I{INCLUDE}", f"-I{CSRC}",
-             f"-I{ROCM}/include", "-D__HIP_PLATFORM_AMD__"]
+CXX_FLAGS = ["-O3", "-std=c++17", "-fPIC", "-Wall", "-fopenmp", f"-I{INCLUDE}", f"-I{CSRC}"]
 
 
 def _sources():
