@@ -88,7 +88,9 @@ typedef struct {
 /* Replaces the cuFFTCC2D / cuICGN2D constructors + cuICGN2D::prepare
  * (TW_paDIC_cuFFTCC2D.cu:192-210, TW_paDIC_cuICGN2D.cu:694-768):
  * validates the geometry (std::logic_error "Number of POIs is below 0!" in
- * the reference becomes TWDIC_EINVAL) and allocates ROI-sized state. */
+ * the reference becomes TWDIC_EINVAL; images are limited to 65535 px per
+ * side and 2^31 bytes, since the kernels use 32-bit image offsets) and
+ * allocates ROI-sized state. */
 twdic_status twdic_create(const twdic_config* cfg, int device, twdic_ctx** out);
 /* Replaces ~cuFFTCC2D / ~cuICGN2D / cuDestroyFFTCC / cuFinalize
  * (TW_paDIC_cuFFTCC2D.cu:212-233,649-677; TW_paDIC_cuICGN2D.cu:719-729,970-989).

@@ -83,6 +83,11 @@ def test_create_without_gpu_fails_cleanly(lib_built):
     assert lib_built.twdic_create(C.byref(bad), 0, C.byref(h)) == api.EINVAL
     bad = api.make_config(64, 64, 1, 1, 62, 62, 5, 5, 5, 5, 2, 2, 33)   # odd window
     assert lib_built.twdic_create(C.byref(bad), 0, C.byref(h)) == api.EINVAL
+    # beyond the 32-bit image addressing: 65536 px wide, or 2^31 bytes
+    bad = api.make_config(65536, 64, 1, 1, 62, 62, 5, 5, 5, 5, 2, 2, 16)
+    assert lib_built.twdic_create(C.byref(bad), 0, C.byref(h)) == api.EINVAL
+    bad = api.make_config(46341, 46341, 1, 1, 62, 62, 5, 5, 5, 5, 2, 2, 16)
+    assert lib_built.twdic_create(C.byref(bad), 0, C.byref(h)) == api.EINVAL
 
 
 def test_num_poi_matches_oracle(lib_built):

@@ -108,6 +108,8 @@ const char* check_pair(const twdic_pair& p)
 {
     if (!p.d_ref || !p.d_tar || (p.n > 0 && (!p.d_poi_yx || !p.d_u || !p.d_v))) return "null pointer in pair";
     if (p.img_w <= 0 || p.img_h <= 0 || p.stride < p.img_w) return "bad image size/stride";
+    if (p.stride > 65535 || p.img_h > 65535 || (int64_t)p.stride * p.img_h >= (int64_t)1 << 31)
+        return "images are limited to 65535 px per side (stride included) and 2^31 bytes";
     if (p.roi_x < 1 || p.roi_y < 1 || p.roi_w < 2 || p.roi_h < 2 || p.roi_x + p.roi_w + 1 > p.img_w ||
         p.roi_y + p.roi_h + 1 > p.img_h)
         return "ROI needs a one-pixel frame inside the image";

@@ -448,7 +448,8 @@ twdic_status twdic_server_create(const twdic_server_config* cfg, twdic_server** 
     if (!cfg || !out) return TWDIC_EINVAL;
     *out = nullptr;
     if (cfg->slots < 1 || cfg->slots > 64 || cfg->queue_capacity < 1 || cfg->max_width < 4 ||
-        cfg->max_height < 4 || cfg->max_pois < 1)
+        cfg->max_height < 4 || cfg->max_pois < 1 || cfg->max_width > 65535 || cfg->max_height > 65535 ||
+        (int64_t)cfg->max_width * cfg->max_height >= (int64_t)1 << 31)
         return TWDIC_EINVAL;
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0 || cfg->device < 0 || cfg->device >= ndev)

@@ -48,6 +48,9 @@ void num_poi(const twdic_config& c, int* nx, int* ny)
 const char* validate(const twdic_config& c)
 {
     if (c.img_w <= 0 || c.img_h <= 0) return "image size must be positive";
+    // kernels address images with 32-bit byte offsets and 24-bit row products
+    if (c.img_w > 65535 || c.img_h > 65535 || (int64_t)c.img_w * c.img_h >= (int64_t)1 << 31)
+        return "images are limited to 65535 px per side and 2^31 bytes";
     if (c.roi_w < 2 || c.roi_h < 2) return "ROI must be at least 2x2";
     if (c.roi_x < 0 || c.roi_y < 0 || c.roi_x + c.roi_w > c.img_w || c.roi_y + c.roi_h > c.img_h)
         return "ROI must lie inside the image";
