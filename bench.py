@@ -32,6 +32,7 @@ sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBS = 8000.0
 FP32_PEAK_TFLOPS = 157.3
+FP32_UNPACKED_PEAK_TFLOPS = 78.6    # without packed FP32 (v_pk_fma_f32), which this build does not use
 INT8_MFMA_PEAK_TOPS = 5000.0
 
 
@@ -416,7 +417,9 @@ def main():
                             "frac": d["gbs"] / HBM_PEAK_GBS, "algorithmic_bytes_per_launch": d["bytes"]}}
         return {"bound": "mfma", "pipe": "fp32 VALU (no MFMA: no dense contraction on this path)",
                 "kernel": name, "achieved": d["tflops"], "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
-                "frac": d["tflops"] / FP32_PEAK_TFLOPS, "traffic": d["traffic"],
+                "frac": d["tflops"] / FP32_PEAK_TFLOPS,
+                "peak_without_packed_fp32": FP32_UNPACKED_PEAK_TFLOPS,
+                "frac_without_packed_fp32": d["tflops"] / FP32_UNPACKED_PEAK_TFLOPS, "traffic": d["traffic"],
                 "avg_launch_ms": d["ms"], "algorithmic_flops_per_launch": d["flops"],
                 "hbm": {"achieved": d["gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": d["gbs"] / HBM_PEAK_GBS,
                         "algorithmic_bytes_per_launch": d["bytes"]}}
