@@ -92,3 +92,18 @@ def test_flat_and_outside_windows(N):
     for i in (0, 3, 4):
         assert g["u"][i] == 0 and g["v"][i] == 0 and g["zncc"][i] == 0
     _check(ref, tar, poi[:3], N)
+
+
+@pytest.mark.parametrize("N", [2, 6, 30, 34, 40, 46, 48, 50, 62])
+def test_every_row_alignment_and_image_corner(N):
+    """Windows whose rows start at each of the 16 byte offsets within an
+    aligned 16-byte chunk (the engine reads window rows as the aligned chunks
+    that hold them; the odd image width also varies the offset from row to
+    row), and one window that ends at the image's last pixel."""
+    rng = np.random.default_rng(300 + N)
+    h, w = 2 * N + 40, 2 * N + 57
+    ref = rng.integers(0, 256, (h, w)).astype(np.uint8)
+    tar = np.roll(ref, (1, -2), (0, 1))
+    poi = [(N // 2 + 3 + i % 3, N // 2 + i) for i in range(16)] + [(h - N // 2, w - N // 2)]
+    g, ex = _check(ref, tar, np.array(poi, np.int32), N)
+    assert np.all(ex["status"] == oracle.FFTCC_OK)

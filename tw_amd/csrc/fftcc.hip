@@ -15,8 +15,9 @@
 // on the engine, batching or pairing.
 //
 // Engines:
-//   * k_fftcc_mma<NT> (default, any even N <= 64): the whole integer
-//     correlation surface on the int8 matrix cores, one wavefront per POI.
+//   * k_fftcc_mma<NT, N64, TILE> (default, any even N <= 64): the whole
+//     integer correlation surface on the int8 matrix cores, one wavefront per
+//     POI (32 x 32 x 32 tiles; 16 x 16 x 64 tiles for 32 < N <= 48).
 //   * k_fftcc_fft<P> (N = 4P = 32 or 64): FP32 FFTs.  One workgroup of 4N
 //     threads owns a POI pair.  Each window pair is packed into
 //     Z = R^ + i T^ (R^, T^ zero-mean, unit norm), so one forward 2-D FFT
@@ -740,7 +741,8 @@ __global__ __launch_bounds__(kDftThreads) void k_batch_fftcc_dft(const PairDesc*
 // circular cross-correlation C[kv][ku] = sum_{y,x} R[y][x] T[(y+kv)%N][(x+ku)%N],
 // up to the zero-mean constant and rounding.  This engine computes it
 // directly, exactly, in integers on the int8 matrix cores
-// (v_mfma_i32_32x32x32_i8).  Stored as signed bytes R' = R - 128 and
+// (v_mfma_i32_32x32x32_i8, or v_mfma_i32_16x16x64_i8 for 32 < N <= 48).
+// Stored as signed bytes R' = R - 128 and
 // T' = T - 128, C'[k] = C[k] - 128 (sum R + sum T) + 128^2 N^2 differs from
 // C[k] by a constant, so the argmax (lowest row-major index on ties) is the
 // exact integer peak, and |C'| <= 2^14 N^2 <= 2^26 fits int32.
@@ -750,8 +752,9 @@ __global__ __launch_bounds__(kDftThreads) void k_batch_fftcc_dft(const PairDesc*
 // A[kv][(y', x)] gathers reference rows (the row index turns with y');
 // B[(y', x)][ku] is the circulant of target row y', read from a doubled copy
 // of the row, T2[c] = T'[c mod N], at byte offset x + ku.  Outputs are
-// NT x NT (NT = 32 for N <= 32, else 64) in 32 x 32 tiles; rows and columns
-// of K or of the output beyond N are zero padding or ignored.
+// NT x NT (NT = 32 for N <= 32, else 64) in 32 x 32 tiles, or 48 x 48 in
+// 16 x 16 tiles; rows and columns of K or of the output beyond N are zero
+// padding or ignored.
 //
 // Lane layout (one wavefront, lane l, m = l % 32, h = l / 32): an A operand
 // holds row m, bytes 16h .. 16h+15 of a 32-byte K chunk; a B operand holds
@@ -775,6 +778,25 @@ struct MmaSmem {
     alignas(16) uint8_t t[NT * TS];
 };
 
+// 32 < N <= 48 on 16 x 16 output tiles: 48 rows; the target row is read up
+// to byte 111 (offset 16 g + 16 t' + n <= 95, plus 16 bytes)
+struct MmaSmem16 {
+    static constexpr int RS = 80;
+    static constexpr int TW = 112;
+    static constexpr int TS = 112;                    // 28 dwords: 8 consecutive rows in distinct bank groups
+    alignas(16) uint8_t r[48 * RS];
+    alignas(16) uint8_t t[48 * TS];
+};
+
+template <int NT, bool N64, int TILE>
+struct MmaSel {
+    using type = MmaSmem<NT, N64>;
+};
+template <>
+struct MmaSel<64, false, 16> {
+    using type = MmaSmem16;
+};
+
 // 16 bytes of a doubled target row starting at byte offset o (any alignment)
 __device__ __forceinline__ v4i b_operand(const uint32_t* row, int d, int sh)
 {
@@ -787,12 +809,53 @@ __device__ __forceinline__ v4i b_operand(const uint32_t* row, int d, int sh)
     return b;
 }
 
-// N64: N == 64 (NT = 64); otherwise N <= NT is a run-time value
-template <int NT, bool N64>
+// The n bytes (2 <= n <= 4 WORDS) at src, any alignment, into w
+// (bytes from n on are unspecified).  Only the 16-byte-aligned chunks that
+// hold at least one of the n bytes are read; an aligned chunk never crosses a
+// page, so nothing beyond the pages of the row itself is touched.  The
+// misalignment sh (0..15) is taken out as whole dwords in two select steps,
+// then as bytes with v_alignbyte.
+template <int WORDS>
+__device__ __forceinline__ void load_row(const uint8_t* src, int n, uint32_t (&w)[WORDS])
+{
+    constexpr int CH = WORDS / 4 + 1;                  // raw words 0 .. WORDS + 3 are used
+    const uintptr_t a = reinterpret_cast<uintptr_t>(src);
+    const uint4* c = reinterpret_cast<const uint4*>(a & ~(uintptr_t)15);
+    const int sh = (int)(a & 15);
+    uint32_t raw[4 * CH];
+#pragma unroll
+    for (int k = 0; k < CH; ++k) {
+        uint4 v = make_uint4(0u, 0u, 0u, 0u);
+        if (16 * k < sh + n) v = c[k];
+        raw[4 * k] = v.x;
+        raw[4 * k + 1] = v.y;
+        raw[4 * k + 2] = v.z;
+        raw[4 * k + 3] = v.w;
+    }
+    // bitwise selects on opaque masks (v_bfi_b32): a `q ? raw[i + 2] : raw[i]`
+    // is folded into a dynamically indexed load, which moves raw to scratch
+    uint32_t m2 = (sh & 8) ? ~0u : 0u, m1 = (sh & 4) ? ~0u : 0u;
+    asm volatile("" : "+v"(m2), "+v"(m1));
+    const int b = sh & 3;
+    uint32_t s2[WORDS + 2], s1[WORDS + 1];
+#pragma unroll
+    for (int i = 0; i < WORDS + 2; ++i) s2[i] = (raw[i + 2] & m2) | (raw[i] & ~m2);
+#pragma unroll
+    for (int i = 0; i < WORDS + 1; ++i) s1[i] = (s2[i + 1] & m1) | (s2[i] & ~m1);
+#pragma unroll
+    for (int k = 0; k < WORDS; ++k) w[k] = __builtin_amdgcn_alignbyte(s1[k + 1], s1[k], b);
+}
+
+// N64: N == 64 (NT = 64); otherwise N <= NT is a run-time value.  TILE 32:
+// v_mfma_i32_32x32x32_i8 on 32 x 32 tiles (N <= 32 in one tile, 32 < N <= 64
+// in 2 x 2); TILE 16: v_mfma_i32_16x16x64_i8 on 3 x 3 tiles of 16 x 16 with
+// the whole 64-byte row as one K chunk, for 32 < N <= 48, where it issues
+// 9 16-cycle MFMAs per row instead of 8 32-cycle ones.
+template <int NT, bool N64, int TILE>
 __global__ __launch_bounds__(64) void k_fftcc_mma(FftccParams p)
 {
-    constexpr int TT = NT / 32;          // output tiles per edge and K chunks per row
-    using Sm = MmaSmem<NT, N64>;
+    constexpr int TT = NT / 32;          // output tiles per edge and K chunks per row (TILE 32)
+    using Sm = typename MmaSel<NT, N64, TILE>::type;
     __shared__ Sm sm;
     const int poi = blockIdx.x;
     const int lane = threadIdx.x, m = lane & 31, h = lane >> 5;
@@ -833,28 +896,47 @@ __global__ __launch_bounds__(64) void k_fftcc_mma(FftccParams p)
                                    t[4 * q + 3] ^ 0x80808080u);
             }
         } else {
-            // any even N < NT: bytewise stores only (reference bytes x >= N are
-            // zero, the target row repeats with period N), then the sums from
-            // the signed bytes with explicit dot products.  (A guarded byte sum
-            // `if (x < N) s += a * a` was folded by the compiler into a v_dot4
-            // with a wrong accumulator, adding 1 per row.)
-            for (int x = 0; x < NT; ++x) rrow[x] = x < N ? (uint8_t)(p.ref[o + x] ^ 0x80) : (uint8_t)0;
-            for (int c = 0, x = 0; c < Sm::TW; ++c) {
-                trow[c] = (uint8_t)(p.tar[o + x] ^ 0x80);
-                if (++x == N) x = 0;
-            }
-            const int* r32 = reinterpret_cast<const int*>(rrow);
-            const int* t32 = reinterpret_cast<const int*>(trow);
+            // any even N < NT: the rows as words (load_row), signed, reference
+            // bytes x >= N zeroed, then the sums from the signed words with
+            // explicit dot products.  (A guarded byte sum `if (x < N) s += a * a`
+            // was folded by the compiler into a v_dot4 with a wrong accumulator,
+            // adding 1 per row.)
+            constexpr int WORDS = NT == 32 ? 8 : TILE == 16 ? 12 : 16;   // N <= 30, 48, 62
+            uint32_t r[WORDS], t[WORDS];
+            load_row<WORDS>(p.ref + o, N, r);
+            load_row<WORDS>(p.tar + o, N, t);
             int s1r = 0, s2r = 0, s1t = 0, s2t = 0;
 #pragma unroll
-            for (int k = 0; k < NT / 4; ++k) {
+            for (int k = 0; k < WORDS; ++k) {
                 const int x = 4 * k;
-                const int keep = x + 4 <= N ? -1 : x >= N ? 0 : (int)((1u << (8 * (N - x))) - 1u);
-                const int wr = r32[k], wt = t32[k] & keep;
+                const uint32_t keep = x + 4 <= N ? ~0u : x >= N ? 0u : 0xffffu;   // N is even
+                r[k] = (r[k] ^ 0x80808080u) & keep;
+                t[k] ^= 0x80808080u;
+                const int wr = (int)r[k], wt = (int)(t[k] & keep);
                 s1r = __builtin_amdgcn_sdot4(wr, 0x01010101, s1r, false);
                 s2r = __builtin_amdgcn_sdot4(wr, wr, s2r, false);
                 s1t = __builtin_amdgcn_sdot4(wt, 0x01010101, s1t, false);
                 s2t = __builtin_amdgcn_sdot4(wt, wt, s2t, false);
+            }
+            uint4* r4 = reinterpret_cast<uint4*>(rrow);
+#pragma unroll
+            for (int k = 0; k < NT / 16; ++k)
+                r4[k] = 4 * k < WORDS ? make_uint4(r[4 * k], r[4 * k + 1], r[4 * k + 2], r[4 * k + 3])
+                                      : make_uint4(0u, 0u, 0u, 0u);
+            // the target row at byte 0 and again at byte N.  Products that count
+            // read bytes x + ku <= 2N - 2 (x, ku < N); later bytes meet zero
+            // reference bytes (x >= N) or output columns ku >= N, which are ignored.
+            uint4* t4 = reinterpret_cast<uint4*>(trow);
+#pragma unroll
+            for (int k = 0; k < WORDS / 4; ++k) t4[k] = make_uint4(t[4 * k], t[4 * k + 1], t[4 * k + 2], t[4 * k + 3]);
+            uint32_t* t32 = reinterpret_cast<uint32_t*>(trow) + (N >> 2);
+            if ((N & 2) == 0) {
+#pragma unroll
+                for (int e = 0; e < WORDS; ++e) t32[e] = t[e];
+            } else {                                       // N = 4 m + 2: bytes 0, 1 at N, then shifted words
+                *reinterpret_cast<uint16_t*>(trow + N) = (uint16_t)t[0];
+#pragma unroll
+                for (int e = 0; e + 1 < WORDS; ++e) t32[1 + e] = __builtin_amdgcn_alignbyte(t[e + 1], t[e], 2);
             }
             // bytes are v - 128: sum v = s1 + 128 N, sum v^2 = s2 + 256 s1 + 16384 N
             sr = s1r + 128 * N;
@@ -871,79 +953,136 @@ __global__ __launch_bounds__(64) void k_fftcc_mma(FftccParams p)
     }
     __syncthreads();
 
-    // -- C' = A B over K = N rows x TT chunks of 32 bytes ------------------------
-    int ra[TT];                                        // A row byte offsets, (y' - kv) mod N
-#pragma unroll
-    for (int t = 0; t < TT; ++t) {
-        const int kv = 32 * t + m;
-        ra[t] = (kv < N ? (kv == 0 ? 0 : N - kv) : 0) * Sm::RS + 16 * h;
-    }
-    // B operand j (j = K chunk + output tile) starts at byte 32 j + 16 h + m
-    constexpr int NB = TT == 1 ? 1 : N64 ? 2 : 3;      // distinct B operands per row
-    int bd[NB], bs[NB];
-#pragma unroll
-    for (int j = 0; j < NB; ++j) {
-        const int o = 32 * j + 16 * h + m;
-        bd[j] = o >> 2;
-        bs[j] = o & 3;
-    }
-    const int rwrap = N * Sm::RS;
-    v16i acc[TT][TT];
-#pragma unroll
-    for (int a = 0; a < TT; ++a)
-#pragma unroll
-        for (int b = 0; b < TT; ++b) acc[a][b] = v16i{};
-
-    // A operands of row y': lane (m, h) holds reference row (y' - kv) mod N,
-    // read from LDS for the next row while the MFMAs of this one run.  (Passing
-    // them down one lane per row instead, with a DPP shift and two LDS reads,
-    // measured 0.8 % slower: the kernel is power-limited, not LDS-bound.)
-    v4i A[TT][TT];                                     // [K chunk][output row tile]
-#pragma unroll
-    for (int xc = 0; xc < TT; ++xc)
-#pragma unroll
-        for (int t = 0; t < TT; ++t) A[xc][t] = *reinterpret_cast<const v4i*>(sm.r + ra[t] + 32 * xc);
-
-#pragma unroll 2
-    for (int y = 0; y < N; ++y) {
-        const uint32_t* trow = reinterpret_cast<const uint32_t*>(sm.t + y * Sm::TS);
-        v4i B[NB];
-#pragma unroll
-        for (int j = 0; j < NB; ++j) B[j] = b_operand(trow, bd[j], bs[j]);
-#pragma unroll
-        for (int xc = 0; xc < TT; ++xc)
-#pragma unroll
-            for (int t = 0; t < TT; ++t)
-#pragma unroll
-                for (int t2 = 0; t2 < TT; ++t2)     // offset 64 is offset 0 when N = 64
-                    acc[t][t2] = __builtin_amdgcn_mfma_i32_32x32x32_i8(A[xc][t], B[(xc + t2) % NB], acc[t][t2], 0, 0, 0);
+    int best = INT_MIN, bk = INT_MAX;
+    if constexpr (TILE == 32) {
+        // -- C' = A B over K = N rows x TT chunks of 32 bytes ------------------------
+        int ra[TT];                                        // A row byte offsets, (y' - kv) mod N
 #pragma unroll
         for (int t = 0; t < TT; ++t) {
-            ra[t] += Sm::RS;
-            if (ra[t] >= rwrap) ra[t] -= rwrap;
+            const int kv = 32 * t + m;
+            ra[t] = (kv < N ? (kv == 0 ? 0 : N - kv) : 0) * Sm::RS + 16 * h;
         }
+        // B operand j (j = K chunk + output tile) starts at byte 32 j + 16 h + m
+        constexpr int NB = TT == 1 ? 1 : N64 ? 2 : 3;      // distinct B operands per row
+        int bd[NB], bs[NB];
+#pragma unroll
+        for (int j = 0; j < NB; ++j) {
+            const int o = 32 * j + 16 * h + m;
+            bd[j] = o >> 2;
+            bs[j] = o & 3;
+        }
+        const int rwrap = N * Sm::RS;
+        v16i acc[TT][TT];
+#pragma unroll
+        for (int a = 0; a < TT; ++a)
+#pragma unroll
+            for (int b = 0; b < TT; ++b) acc[a][b] = v16i{};
+
+        // A operands of row y': lane (m, h) holds reference row (y' - kv) mod N,
+        // read from LDS for the next row while the MFMAs of this one run.  (Passing
+        // them down one lane per row instead, with a DPP shift and two LDS reads,
+        // measured 0.8 % slower: the kernel is power-limited, not LDS-bound.)
+        v4i A[TT][TT];                                     // [K chunk][output row tile]
 #pragma unroll
         for (int xc = 0; xc < TT; ++xc)
 #pragma unroll
             for (int t = 0; t < TT; ++t) A[xc][t] = *reinterpret_cast<const v4i*>(sm.r + ra[t] + 32 * xc);
-    }
 
-    // -- exact peak: largest C', lowest row-major index on ties (:300-311) -------
-    // a lane's values are visited in increasing index order (kv, then ku), so
-    // a strict comparison keeps the lowest index among equal values
-    int best = INT_MIN, bk = INT_MAX;
+#pragma unroll 2
+        for (int y = 0; y < N; ++y) {
+            const uint32_t* trow = reinterpret_cast<const uint32_t*>(sm.t + y * Sm::TS);
+            v4i B[NB];
 #pragma unroll
-    for (int t = 0; t < TT; ++t)
+            for (int j = 0; j < NB; ++j) B[j] = b_operand(trow, bd[j], bs[j]);
 #pragma unroll
-        for (int r = 0; r < 16; ++r)
+            for (int xc = 0; xc < TT; ++xc)
 #pragma unroll
-            for (int t2 = 0; t2 < TT; ++t2) {
-                const int kv = 32 * t + 8 * (r >> 2) + 4 * h + (r & 3), ku = 32 * t2 + m;
-                const int v = acc[t][t2][r];
-                const bool take = kv < N && ku < N && v > best;
-                best = take ? v : best;
-                bk = take ? kv * N + ku : bk;
+                for (int t = 0; t < TT; ++t)
+#pragma unroll
+                    for (int t2 = 0; t2 < TT; ++t2)     // offset 64 is offset 0 when N = 64
+                        acc[t][t2] = __builtin_amdgcn_mfma_i32_32x32x32_i8(A[xc][t], B[(xc + t2) % NB], acc[t][t2], 0, 0, 0);
+#pragma unroll
+            for (int t = 0; t < TT; ++t) {
+                ra[t] += Sm::RS;
+                if (ra[t] >= rwrap) ra[t] -= rwrap;
             }
+#pragma unroll
+            for (int xc = 0; xc < TT; ++xc)
+#pragma unroll
+                for (int t = 0; t < TT; ++t) A[xc][t] = *reinterpret_cast<const v4i*>(sm.r + ra[t] + 32 * xc);
+        }
+
+        // -- exact peak: largest C', lowest row-major index on ties (:300-311) -------
+        // a lane's values are visited in increasing index order (kv, then ku), so
+        // a strict comparison keeps the lowest index among equal values
+#pragma unroll
+        for (int t = 0; t < TT; ++t)
+#pragma unroll
+            for (int r = 0; r < 16; ++r)
+#pragma unroll
+                for (int t2 = 0; t2 < TT; ++t2) {
+                    const int kv = 32 * t + 8 * (r >> 2) + 4 * h + (r & 3), ku = 32 * t2 + m;
+                    const int v = acc[t][t2][r];
+                    const bool take = kv < N && ku < N && v > best;
+                    best = take ? v : best;
+                    bk = take ? kv * N + ku : bk;
+                }
+    } else {
+        // -- 16 x 16 tiles: lane (n, g), n = lane % 16, g = lane / 16 ----------
+        // A tile t: row (y' - kv) mod N, kv = 16 t + n, bytes 16 g .. 16 g + 15
+        // of the 64-byte row (zero beyond N); B tile t': the target row at
+        // byte offset 16 g + 16 t' + n; accumulator register r of lane l is
+        // output (row 4 g + r, column n) of its tile.
+        const int n16 = lane & 15, g = lane >> 4;
+        int ra[3], bd[3], bs[3];
+#pragma unroll
+        for (int t = 0; t < 3; ++t) {
+            const int kv = 16 * t + n16;
+            ra[t] = (kv < N ? (kv == 0 ? 0 : N - kv) : 0) * Sm::RS + 16 * g;
+            const int o = 16 * g + 16 * t + n16;
+            bd[t] = o >> 2;
+            bs[t] = o & 3;
+        }
+        const int rwrap = N * Sm::RS;
+        v4i acc[3][3];
+#pragma unroll
+        for (int a = 0; a < 3; ++a)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) acc[a][c] = v4i{};
+        v4i A[3];
+#pragma unroll
+        for (int t = 0; t < 3; ++t) A[t] = *reinterpret_cast<const v4i*>(sm.r + ra[t]);
+        for (int y = 0; y < N; ++y) {
+            const uint32_t* trow = reinterpret_cast<const uint32_t*>(sm.t + y * Sm::TS);
+            v4i B[3];
+#pragma unroll
+            for (int t = 0; t < 3; ++t) B[t] = b_operand(trow, bd[t], bs[t]);
+#pragma unroll
+            for (int t = 0; t < 3; ++t)
+#pragma unroll
+                for (int t2 = 0; t2 < 3; ++t2)
+                    acc[t][t2] = __builtin_amdgcn_mfma_i32_16x16x64_i8(A[t], B[t2], acc[t][t2], 0, 0, 0);
+#pragma unroll
+            for (int t = 0; t < 3; ++t) {
+                ra[t] += Sm::RS;
+                if (ra[t] >= rwrap) ra[t] -= rwrap;
+                A[t] = *reinterpret_cast<const v4i*>(sm.r + ra[t]);
+            }
+        }
+        // increasing index order within the lane: kv = 16 t + 4 g + r, then ku
+#pragma unroll
+        for (int t = 0; t < 3; ++t)
+#pragma unroll
+            for (int r = 0; r < 4; ++r)
+#pragma unroll
+                for (int t2 = 0; t2 < 3; ++t2) {
+                    const int kv = 16 * t + 4 * g + r, ku = 16 * t2 + n16;
+                    const int v = acc[t][t2][r];
+                    const bool take = kv < N && ku < N && v > best;
+                    best = take ? v : best;
+                    bk = take ? kv * N + ku : bk;
+                }
+    }
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) {
         const int ov = __shfl_xor(best, o), ok = __shfl_xor(bk, o);
@@ -974,11 +1113,13 @@ hipError_t launch_fftcc(const FftccParams& p, int variant, hipStream_t st)
     const int pairs = (p.n + 1) / 2;
     if (variant == kFftccMma) {
         if (p.N == 64)
-            hipLaunchKernelGGL((k_fftcc_mma<64, true>), dim3(p.n), dim3(64), 0, st, p);
+            hipLaunchKernelGGL((k_fftcc_mma<64, true, 32>), dim3(p.n), dim3(64), 0, st, p);
+        else if (p.N > 48)
+            hipLaunchKernelGGL((k_fftcc_mma<64, false, 32>), dim3(p.n), dim3(64), 0, st, p);
         else if (p.N > 32)
-            hipLaunchKernelGGL((k_fftcc_mma<64, false>), dim3(p.n), dim3(64), 0, st, p);
+            hipLaunchKernelGGL((k_fftcc_mma<64, false, 16>), dim3(p.n), dim3(64), 0, st, p);
         else
-            hipLaunchKernelGGL((k_fftcc_mma<32, false>), dim3(p.n), dim3(64), 0, st, p);
+            hipLaunchKernelGGL((k_fftcc_mma<32, false, 32>), dim3(p.n), dim3(64), 0, st, p);
     } else if (p.N == 64) {
         if (variant == 3)
             hipLaunchKernelGGL((k_fftcc_fft<16, 3>), dim3(pairs), dim3(256), 0, st, p);
