@@ -407,7 +407,8 @@ def main():
     def roof(name, d):
         if d.get("int8_ops"):
             tops = d["int8_ops"] / (d["ms"] * 1e-3) / 1e12
-            return {"bound": "mfma", "pipe": "int8 MFMA (v_mfma_i32_32x32x32_i8), exact integer correlation",
+            shape = "16x16x64" if 32 < N <= 48 else "32x32x32"
+            return {"bound": "mfma", "pipe": f"int8 MFMA (v_mfma_i32_{shape}_i8), exact integer correlation",
                     "kernel": name, "achieved": tops, "peak": INT8_MFMA_PEAK_TOPS, "unit": "TOP/s",
                     "frac": tops / INT8_MFMA_PEAK_TOPS, "traffic": d["traffic"],
                     "avg_launch_ms": d["ms"], "algorithmic_ops_per_launch": d["int8_ops"],
@@ -415,7 +416,7 @@ def main():
                                        "algorithmic_flops_per_launch": d["flops"]},
                     "hbm": {"achieved": d["gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
                             "frac": d["gbs"] / HBM_PEAK_GBS, "algorithmic_bytes_per_launch": d["bytes"]}}
-        return {"bound": "mfma", "pipe": "fp32 VALU (no MFMA: no dense contraction on this path)",
+        return {"bound": "mfma", "pipe": "fp32/fp64 VALU (IC-GN has no dense contraction for the matrix cores)",
                 "kernel": name, "achieved": d["tflops"], "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
                 "frac": d["tflops"] / FP32_PEAK_TFLOPS,
                 "peak_without_packed_fp32": FP32_UNPACKED_PEAK_TFLOPS,

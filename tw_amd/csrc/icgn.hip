@@ -94,9 +94,12 @@ __device__ __forceinline__ float interp_fused(const uint8_t* __restrict__ tar, u
     acc = fmaf(wy[1], row_dot(r1, wx), acc);
     acc = fmaf(wy[2], row_dot(r2, wx), acc);
     acc = fmaf(wy[3], row_dot(r3, wx), acc);
-    const uint32_t rep = __builtin_amdgcn_perm(r1, r1, 0x01010101u);   // byte 1 of r1, four times
-    const bool uniform = (r0 == rep) & (r1 == rep) & (r2 == rep) & (r3 == rep);
-    return uniform ? (float)((r1 >> 8) & 255u) : acc;
+    // uniform: rows 0, 2, 3 equal row 1, and row 1 equals itself turned by a
+    // byte.  XORs and ORs issue beside the FP32 work; kept opaque, or the
+    // compiler splits the test back into four compares, which do not.
+    uint32_t diff = (r0 ^ r1) | (r2 ^ r1) | (r3 ^ r1) | (r1 ^ __builtin_amdgcn_alignbit(r1, r1, 8));
+    asm volatile("" : "+v"(diff));
+    return diff == 0u ? (float)((r1 >> 8) & 255u) : acc;
 }
 
 // W(p) o W(dp)^-1 and p from W: TW_paDIC_ICGN2D_CPU.cpp:713-745 / :875-907,
@@ -139,44 +142,47 @@ __device__ __forceinline__ double conv_norm(const float d[6], int Sx, int Sy)
 
 }  // namespace
 
-// Workgroups are dispatched round-robin over the 8 XCDs; remap so each XCD
-// walks one contiguous range of POIs and its L2 holds one band of the image.
-__device__ __forceinline__ int xcd_block(int b, int nb)
+// The subset's pixels in row-major order, p = l * Wsub + m: the exact float
+// offsets dx = m - Sx, dy = l - Sy and the byte offset 4 (l * rw + m) into
+// the gradient plane from the subset's corner.  Every POI of a launch has the
+// same subset and ROI width, so one table in LDS serves a workgroup's four
+// wavefronts, and a lane's step to its next pixel is an LDS read instead of
+// float compares and selects.
+constexpr int kStepTable = 2048;          // pixels: subsets up to 45 x 45
+struct StepTable {
+    float2 d[kStepTable];
+    uint32_t go[kStepTable];
+};
+
+__device__ __forceinline__ void fill_step_table(StepTable& t, const IcgnParams& P)
 {
-    const int x = b & 7, slot = b >> 3, q = nb >> 3, r = nb & 7;
-    return x * q + min(x, r) + slot;
+    for (int p = threadIdx.x; p < P.s; p += blockDim.x) {
+        const int l = p / P.Wsub, m = p - l * P.Wsub;
+        t.d[p] = make_float2((float)(m - P.Sx), (float)(l - P.Sy));
+        t.go[p] = 4u * (uint32_t)(l * P.rw + m);
+    }
 }
 
 // One pass over the subset at the current warp.  INT: first pass at integer
 // target pixels (:610-637); otherwise bicubic (:757-792).  Returns false when
 // a warped pixel leaves the ROI.  Accumulates sum T, sum T^2 and the six
-// sum SD_k T / 2 (the gradient plane stores 2 Rx, 2 Ry) in FP64.
-template <int INTERP, bool INT, bool CHECK>
+// sum SD_k T / 2 (the gradient plane stores 2 Rx, 2 Ry) in FP64.  With a step
+// table (tab != null) the lanes take pixels lane + 64 k from it: s / 64 full
+// rounds, then the s % 64 remaining pixels; without one they step their
+// offsets in registers.  Each lane visits the same pixels in the same order
+// either way, so the sums are identical.
+template <int INTERP, bool INT, bool CHECK, bool TABLE>
 __device__ __forceinline__ bool subset_loop(const IcgnParams& P, int px, int py, const float Wm[6], int lane,
-                                            double acc[8])
+                                            double acc[8], const StepTable* tab)
 {
     const int Sx = P.Sx, Sy = P.Sy, Wsub = P.Wsub, s = P.s;
     const int sx0 = P.sx0, sy0 = P.sy0, rw = P.rw, rh = P.rh;
     const uint32_t stride = (uint32_t)P.stride;
-    // lane walks pixels p = lane, lane + 64, ...: column m = p % Wsub, row
-    // l = p / Wsub, kept as the exact float offsets dx = m - Sx, dy = l - Sy
-    // and the 32-bit byte offset into the gradient plane
-    const int q64 = 64 / Wsub, r64 = 64 - q64 * Wsub;
-    // recomputed per pass: kept live across the iteration loop, this start
-    // state is what the register allocator spills
-    int lane_v = lane;
-    asm volatile("" : "+v"(lane_v));
-    const int l0 = lane_v / Wsub, m0 = lane_v - l0 * Wsub;
-    float dx = (float)(m0 - Sx), dy = (float)(l0 - Sy);
-    uint32_t go = 4u * (uint32_t)((py - Sy - sy0 + l0) * rw + (px - Sx - sx0 + m0));
-    const float fSx = (float)Sx, fWsub = (float)Wsub, fr64 = (float)r64, fq64 = (float)q64;
-    const uint32_t gstep = 4u * (uint32_t)(q64 * rw + r64), gwrap = 4u * (uint32_t)(rw - Wsub);
     const uint8_t* g2b = reinterpret_cast<const uint8_t*>(P.g2);
     const float fpx = (float)px, fpy = (float)py;
     double st = 0.0, stt = 0.0, a0 = 0.0, a1 = 0.0, a2 = 0.0, b0 = 0.0, b1 = 0.0, b2 = 0.0;
     bool out = false;
-#pragma unroll 1
-    for (int p = lane; p < s; p += 64) {
+    auto pixel = [&](float dx, float dy, uint32_t go) {
         const float wx = warp_coord(fpx, Wm[0], Wm[1], Wm[2], dx, dy);
         const float wy = warp_coord(fpy, Wm[3], Wm[4], Wm[5], dx, dy);
         const int ix = (int)wx, iy = (int)wy;          // truncation, as int()
@@ -193,12 +199,16 @@ __device__ __forceinline__ bool subset_loop(const IcgnParams& P, int px, int py,
         } else if (interior) {
             // :771-772; wx, wy >= 1 here, so x - (float)(int)x is fract(x), exactly
             const float tx = __builtin_amdgcn_fractf(wx), ty = __builtin_amdgcn_fractf(wy);
-            if (INTERP != TWDIC_INTERP_FUSED)
-                T = interp_lut(P.lut, (uint32_t)((iy - sy0) * rw + (ix - sx0)), tx, ty);
-            else
+            if (INTERP == TWDIC_INTERP_FUSED)
                 T = interp_fused(P.tar, stride, ix, iy, tx, ty);
+            else
+                T = interp_lut(P.lut, (uint32_t)((iy - sy0) * rw + (ix - sx0)), tx, ty);
         }
         const uint32_t g = *reinterpret_cast<const uint32_t*>(g2b + go);
+        // FP64: every product SD_k T is exact, and where T takes integer values
+        // (the integer pass, uniform target regions) the sums are exact, so
+        // T = R gives an exactly zero update and near-flat subsets, whose
+        // residual is scaled by the large R_s / T_s, see no rounding there
         const double gx = (double)(int16_t)(g & 0xffffu), gy = (double)(int16_t)(g >> 16);
         const double Td = T, ddx = dx, ddy = dy;
         st += Td;
@@ -210,12 +220,46 @@ __device__ __forceinline__ bool subset_loop(const IcgnParams& P, int px, int py,
         b0 += b;
         b1 = fma(b, ddx, b1);
         b2 = fma(b, ddy, b2);
-        // next pixel of this lane: 64 further in row-major subset order
-        dx += fr64;
-        const bool wrap = dx > fSx;
-        dx = wrap ? dx - fWsub : dx;
-        dy += wrap ? fq64 + 1.f : fq64;
-        go += wrap ? gstep + gwrap : gstep;
+    };
+    // byte offset of the subset's corner pixel in the gradient plane
+    const uint32_t gcorner = 4u * (uint32_t)((py - Sy - sy0) * rw + (px - Sx - sx0));
+    if constexpr (TABLE) {
+        const int full = s >> 6;
+#pragma unroll 1
+        for (int k = 0; k < full; ++k) {
+            const int p = lane + 64 * k;
+            const float2 d = tab->d[p];
+            pixel(d.x, d.y, gcorner + tab->go[p]);
+        }
+        const int p = lane + 64 * full;
+        if (p < s) {
+            const float2 d = tab->d[p];
+            pixel(d.x, d.y, gcorner + tab->go[p]);
+        }
+    } else {
+        // lane walks pixels p = lane, lane + 64, ...: column m = p % Wsub, row
+        // l = p / Wsub, kept as the exact float offsets dx = m - Sx, dy = l - Sy
+        // and the 32-bit byte offset into the gradient plane
+        const int q64 = 64 / Wsub, r64 = 64 - q64 * Wsub;
+        // recomputed per pass: kept live across the iteration loop, this start
+        // state is what the register allocator spills
+        int lane_v = lane;
+        asm volatile("" : "+v"(lane_v));
+        const int l0 = lane_v / Wsub, m0 = lane_v - l0 * Wsub;
+        float dx = (float)(m0 - Sx), dy = (float)(l0 - Sy);
+        uint32_t go = gcorner + 4u * (uint32_t)(l0 * rw + m0);
+        const float fSx = (float)Sx, fWsub = (float)Wsub, fr64 = (float)r64, fq64 = (float)q64;
+        const uint32_t gstep = 4u * (uint32_t)(q64 * rw + r64), gwrap = 4u * (uint32_t)(rw - Wsub);
+#pragma unroll 1
+        for (int p = lane; p < s; p += 64) {
+            pixel(dx, dy, go);
+            // next pixel of this lane: 64 further in row-major subset order
+            dx += fr64;
+            const bool wrap = dx > fSx;
+            dx = wrap ? dx - fWsub : dx;
+            dy += wrap ? fq64 + 1.f : fq64;
+            go += wrap ? gstep + gwrap : gstep;
+        }
     }
     acc[0] = st; acc[1] = stt;
     acc[2] = 0.5 * a0; acc[3] = 0.5 * a1; acc[4] = 0.5 * a2;
@@ -232,9 +276,9 @@ __device__ __forceinline__ bool subset_loop(const IcgnParams& P, int px, int py,
 // region where no per-pixel test can fire (the margin covers FP32 rounding
 // of the per-pixel warp, which is below 1e-3 px), the pass runs without the
 // per-pixel tests; the results are identical either way.
-template <int INTERP, bool INT>
+template <int INTERP, bool INT, bool TABLE>
 __device__ __forceinline__ bool subset_pass(const IcgnParams& P, int px, int py, const float Wm[6], int lane,
-                                            double acc[8])
+                                            double acc[8], const StepTable* tab)
 {
     const float ex = (float)P.Sx, ey = (float)P.Sy, fx = (float)px, fy = (float)py;
     const float x0 = warp_coord(fx, Wm[0], Wm[1], Wm[2], -ex, -ey), x1 = warp_coord(fx, Wm[0], Wm[1], Wm[2], ex, -ey);
@@ -245,14 +289,14 @@ __device__ __forceinline__ bool subset_pass(const IcgnParams& P, int px, int py,
                       fmaxf(fmaxf(x0, x1), fmaxf(x2, x3)) < (float)(P.sx0 + P.rw - 2) &&
                       fminf(fminf(y0, y1), fminf(y2, y3)) >= (float)(P.sy0 + 1) &&
                       fmaxf(fmaxf(y0, y1), fmaxf(y2, y3)) < (float)(P.sy0 + P.rh - 2);
-    if (safe) return subset_loop<INTERP, INT, false>(P, px, py, Wm, lane, acc);
-    return subset_loop<INTERP, INT, true>(P, px, py, Wm, lane, acc);
+    if (safe) return subset_loop<INTERP, INT, false, TABLE>(P, px, py, Wm, lane, acc, tab);
+    return subset_loop<INTERP, INT, true, TABLE>(P, px, py, Wm, lane, acc, tab);
 }
 
 // IC-GN for one POI by one wavefront.  poi is wave-uniform, so per-POI state
 // is read with scalar loads into SGPRs.
-template <int INTERP>
-__device__ __forceinline__ void icgn_poi(const IcgnParams& P, int poi, int lane)
+template <int INTERP, bool TABLE>
+__device__ __forceinline__ void icgn_poi(const IcgnParams& P, int poi, int lane, const StepTable* tab)
 {
     const int py = __builtin_amdgcn_readfirstlane(P.poi[2 * poi]);
     const int px = __builtin_amdgcn_readfirstlane(P.poi[2 * poi + 1]);
@@ -275,8 +319,8 @@ __device__ __forceinline__ void icgn_poi(const IcgnParams& P, int poi, int lane)
         for (int it = 0;; ++it) {
             if (it >= 1) iters = it + 1;                       // ++iNumIterations (:753)
             double acc[8];
-            const bool inside = it == 0 ? subset_pass<INTERP, true>(P, px, py, Wm, lane, acc)
-                                        : subset_pass<INTERP, false>(P, px, py, Wm, lane, acc);
+            const bool inside = it == 0 ? subset_pass<INTERP, true, TABLE>(P, px, py, Wm, lane, acc, tab)
+                                        : subset_pass<INTERP, false, TABLE>(P, px, py, Wm, lane, acc, tab);
             if (!inside) { flag = TWDIC_ICGN_OUT_OF_ROI; break; }         // :632-635, :787-790
 #pragma unroll
             for (int i = 0; i < 8; ++i) acc[i] = wave_sum(acc[i]);
@@ -316,6 +360,13 @@ __device__ __forceinline__ void icgn_poi(const IcgnParams& P, int poi, int lane)
             const float den = warp_den(d);
             if (fabsf(den) <= FLT_EPSILON) { flag = TWDIC_ICGN_SINGULAR_WARP; break; }        // :875-877
             warp_update(Pp, d, den, Wm);
+            // every lane holds the same warp (the butterflies leave the sums
+            // bit-identical in all lanes): keep it in SGPRs for the next pass
+#pragma unroll
+            for (int i = 0; i < 6; ++i) {
+                Wm[i] = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(Wm[i])));
+                Pp[i] = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(Pp[i])));
+            }
             const double nrm = conv_norm(d, Sx, Sy);
             dpn = (float)nrm;
             if (it == 0) iters = 1;                                    // :749
@@ -337,12 +388,31 @@ __device__ __forceinline__ void icgn_poi(const IcgnParams& P, int poi, int lane)
 
 // MINW: occupancy target (waves per SIMD) the register budget is fitted to;
 // 0 leaves the choice to the compiler.
-template <int INTERP, int MINW>
+// TABLE: the subset has at most kStepTable pixels; the workgroup builds the
+// step table in LDS first.
+// One wavefront per POI, four per workgroup; consecutive workgroups go to the
+// 8 XCDs round-robin, so block b is remapped to take POIs from XCD b % 8's
+// contiguous share and each XCD's L2 holds one band of the image.
+__device__ __forceinline__ int xcd_block(int b, int nb)
+{
+    const int x = b & 7, slot = b >> 3, q = nb >> 3, r = nb & 7;
+    return x * q + min(x, r) + slot;
+}
+
+template <int INTERP, int MINW, bool TABLE>
 __global__ __launch_bounds__(256, MINW > 0 ? MINW : 1) void k_icgn(IcgnParams P)
 {
     const int poi = __builtin_amdgcn_readfirstlane(xcd_block(blockIdx.x, gridDim.x) * 4 + (threadIdx.x >> 6));
-    if (poi >= P.n) return;
-    icgn_poi<INTERP>(P, poi, threadIdx.x & 63);
+    if constexpr (TABLE) {
+        __shared__ StepTable tab;
+        fill_step_table(tab, P);
+        __syncthreads();
+        if (poi >= P.n) return;
+        icgn_poi<INTERP, true>(P, poi, threadIdx.x & 63, &tab);
+    } else {
+        if (poi >= P.n) return;
+        icgn_poi<INTERP, false>(P, poi, threadIdx.x & 63, nullptr);
+    }
 }
 
 // Persistent form: every wavefront takes `chunk` consecutive work items
@@ -363,7 +433,7 @@ __global__ __launch_bounds__(256) void k_batch_icgn(const PairDesc* __restrict__
             const int2 it = items[i];
             const int pair = __builtin_amdgcn_readfirstlane(it.x), poi = __builtin_amdgcn_readfirstlane(it.y);
             const IcgnParams P = const_pairs(pairs)[pair].icgn;
-            icgn_poi<TWDIC_INTERP_FUSED>(P, poi, lane);
+            icgn_poi<TWDIC_INTERP_FUSED, false>(P, poi, lane, nullptr);
         }
     }
 }
@@ -384,27 +454,36 @@ hipError_t launch_batch_icgn(const PairDesc* pairs, const int2* items, int total
     return hipGetLastError();
 }
 
-template <int INTERP>
-static void launch_icgn_t(const IcgnParams& p, int variant, dim3 grid, hipStream_t st)
+template <int INTERP, bool TABLE>
+static void launch_icgn_v(const IcgnParams& p, int variant, hipStream_t st)
 {
+    const dim3 grid((p.n + 3) / 4);
     switch (variant) {
-    case 5: hipLaunchKernelGGL((k_icgn<INTERP, 5>), grid, dim3(256), 0, st, p); break;
-    case 6: hipLaunchKernelGGL((k_icgn<INTERP, 6>), grid, dim3(256), 0, st, p); break;
-    case 7: hipLaunchKernelGGL((k_icgn<INTERP, 7>), grid, dim3(256), 0, st, p); break;
-    default: hipLaunchKernelGGL((k_icgn<INTERP, 0>), grid, dim3(256), 0, st, p); break;
+    case 5: hipLaunchKernelGGL((k_icgn<INTERP, 5, TABLE>), grid, dim3(256), 0, st, p); break;
+    case 6: hipLaunchKernelGGL((k_icgn<INTERP, 6, TABLE>), grid, dim3(256), 0, st, p); break;
+    case 7: hipLaunchKernelGGL((k_icgn<INTERP, 7, TABLE>), grid, dim3(256), 0, st, p); break;
+    default: hipLaunchKernelGGL((k_icgn<INTERP, 0, TABLE>), grid, dim3(256), 0, st, p); break;
     }
+}
+
+template <int INTERP>
+static void launch_icgn_t(const IcgnParams& p, int variant, hipStream_t st)
+{
+    if (p.s <= kStepTable)
+        launch_icgn_v<INTERP, true>(p, variant, st);
+    else
+        launch_icgn_v<INTERP, false>(p, variant, st);
 }
 
 hipError_t launch_icgn(const IcgnParams& p, int interp, int variant, hipStream_t st)
 {
     if (p.n <= 0) return hipSuccess;
-    dim3 grid((p.n + 3) / 4);
     if (interp == TWDIC_INTERP_FUSED)
-        launch_icgn_t<TWDIC_INTERP_FUSED>(p, variant, grid, st);
+        launch_icgn_t<TWDIC_INTERP_FUSED>(p, variant, st);
     else if (interp == TWDIC_INTERP_BSPLINE)
-        launch_icgn_t<TWDIC_INTERP_BSPLINE>(p, variant, grid, st);
+        launch_icgn_t<TWDIC_INTERP_BSPLINE>(p, variant, st);
     else
-        launch_icgn_t<TWDIC_INTERP_LUT>(p, variant, grid, st);
+        launch_icgn_t<TWDIC_INTERP_LUT>(p, variant, st);
     return hipGetLastError();
 }
 
