@@ -236,6 +236,8 @@ def parse():
                     help="untimed frames first: kernel times settle only after about a dozen frames")
     ap.add_argument("--interp", choices=["fused", "lut"], default="fused")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-reference-window", action="store_true",
+                    help="skip the secondary frame timing at the reference's FFT-CC window N = 2S")
     ap.add_argument("--cpu-sample", type=int, default=0,
                     help="POIs in the CPU baseline sample (0: the whole frame, about 15 s on 16 cores)")
     ap.add_argument("--workload", choices=["frame", "serving", "batch"], default="frame",
@@ -473,10 +475,59 @@ def main():
         "frame_roofline": frame_roof,
         "synth_s": t_gen,
     }
+    if not a.no_reference_window:
+        out["reference_window"] = reference_window(case, interp, dref, dtar, W, H)
     if not a.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(case, pyx, a.cpu_sample)
     ctx.close()
     print(json.dumps(out))
+
+
+def reference_window(case, interp, dref, dtar, W, H, warmup=20, steps=50):
+    """The same frame with the reference's own FFT-CC window N = 2S (40 for
+    41 x 41 subsets; TW_paDIC_FFTCC2D_CPU.cpp:24-25) instead of 64: a
+    secondary figure beside the headline, which is quoted on N = 64."""
+    from tw_amd import api
+    N = 2 * case.S
+    ctx = api.Context(case.config(interp=interp, N=N))
+    ctx.set_reference_device(dref, W)
+    ctx.set_target_device(dtar, W)
+    poi_d = ctx.poi_grid()
+    n = ctx.nx * ctx.ny
+    u, v, z = (ctx.array(n, np.float32).fill(0) for _ in range(3))
+    p6 = ctx.array((n, 6), np.float32).fill(0)
+    it = ctx.array(n, np.int32).fill(0)
+    fl = ctx.array(n, np.uint8).fill(255)
+
+    def step():
+        ctx.set_target_device(dtar, W)
+        ctx.frame(poi_d, n, u, v, z, p6, it, fl)
+
+    for _ in range(warmup):
+        step()
+    ctx.sync()
+    ctx.record(0)
+    for _ in range(steps):
+        step()
+    ctx.record(1)
+    ctx.sync()
+    ms = ctx.elapsed_ms(0, 1) / steps
+    t_fft, t_icgn = [], []
+    for _ in range(steps):
+        ctx.set_target_device(dtar, W)
+        ctx.record(2)
+        ctx.fftcc(poi_d, n, u, v, z)
+        ctx.record(3)
+        ctx.icgn(poi_d, n, u, v, p6, it, fl)
+        ctx.record(4)
+        ctx.sync()
+        t_fft.append(ctx.elapsed_ms(2, 3))
+        t_icgn.append(ctx.elapsed_ms(3, 4))
+    git, gfl = it.download(), fl.download()
+    ctx.close()
+    return {"fft_window": N, "ms_per_step": ms, "pois_per_s": n / (ms * 1e-3), "fftcc_ms": float(np.mean(t_fft)),
+            "icgn_ms": float(np.mean(t_icgn)), "mean_iterations": float(git[gfl == 0].mean()),
+            "note": "same pair, POIs and subsets; FFT-CC window N = 2S as in the reference"}
 
 
 if __name__ == "__main__":
