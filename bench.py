@@ -32,7 +32,7 @@ sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBS = 8000.0
 FP32_PEAK_TFLOPS = 157.3
-FP32_UNPACKED_PEAK_TFLOPS = 78.6    # without packed FP32 (v_pk_fma_f32), which this build does not use
+FP32_UNPACKED_PEAK_TFLOPS = 78.6    # without packed FP32 (v_pk_fma_f32), which the kernels here do not use
 INT8_MFMA_PEAK_TOPS = 5000.0
 
 

@@ -37,7 +37,7 @@ typedef struct {
     int max_pois;           /* largest POI set accepted */
 } twdic_server_config;
 
-/* Where a request's target and output arrays live.  This build serves host
+/* Where a request's target and output arrays live.  The server takes host
  * memory only (copied through each slot's pinned staging); the other kinds
  * are rejected with TWDIC_EINVAL. */
 enum {
