@@ -173,6 +173,23 @@ twdic_status twdic_frame(twdic_ctx* ctx, const int32_t* d_poi_yx, int n,
                          float* d_u, float* d_v, float* d_zncc, float* d_p6,
                          int32_t* d_iters, uint8_t* d_flags);
 
+/* Graph-captured frame step (extension; the reference launches each frame's
+ * kernels one by one).  twdic_frame_capture records the frame over these
+ * buffers into a HIP graph the context holds: the target LUT in the LUT and
+ * B-spline modes, then FFT-CC and IC-GN exactly as twdic_frame.  The
+ * reference gradients and the per-POI reference state are built before
+ * capture, not recorded.  twdic_frame_replay launches the graph on the
+ * context stream after rebuilding that state if the reference changed; its
+ * results are twdic_frame's on the current target.  New frames go in through
+ * twdic_set_target, which copies into the context's own buffer.  The POI and
+ * output arrays must stay allocated while the graph is held; replay returns
+ * TWDIC_EINVAL when a later call has reallocated the context's per-POI state
+ * (capture again then).  Capturing again replaces the graph. */
+twdic_status twdic_frame_capture(twdic_ctx* ctx, const int32_t* d_poi_yx, int n,
+                                 float* d_u, float* d_v, float* d_zncc, float* d_p6,
+                                 int32_t* d_iters, uint8_t* d_flags);
+twdic_status twdic_frame_replay(twdic_ctx* ctx);
+
 /* ------------------------------------------------------------ POI tracking */
 
 /* The tracking step TW_Core runs around a reference reset

@@ -101,6 +101,8 @@ _SIGS = {
     "twdic_fftcc": ([_VP, _VP, _I, _VP, _VP, _VP, _VP], _I),
     "twdic_icgn": ([_VP, _VP, _I, _VP, _VP, _VP, _VP, _VP, _VP], _I),
     "twdic_frame": ([_VP, _VP, _I, _VP, _VP, _VP, _VP, _VP, _VP], _I),
+    "twdic_frame_capture": ([_VP, _VP, _I, _VP, _VP, _VP, _VP, _VP, _VP], _I),
+    "twdic_frame_replay": ([_VP], _I),
     "twdic_reset_poi_state": ([_VP], _I),
     "twdic_get_buffer": ([_VP, _I, C.POINTER(_VP), C.POINTER(_SZ)], _I),
     "twdic_precompute_target": ([_VP], _I),
@@ -325,6 +327,16 @@ class Context:
         g = lambda a: a.ptr if a is not None else None  # noqa: E731
         self._check(lib().twdic_frame(self.h, poi.ptr, n, u.ptr, v.ptr, zncc.ptr, g(p6), g(iters), g(flags)),
                     "frame")
+
+    def frame_capture(self, poi, n, u, v, zncc, p6=None, iters=None, flags=None):
+        """Record the frame step over these buffers into the context's HIP graph."""
+        g = lambda a: a.ptr if a is not None else None  # noqa: E731
+        self._check(lib().twdic_frame_capture(self.h, poi.ptr, n, u.ptr, v.ptr, zncc.ptr, g(p6), g(iters),
+                                              g(flags)), "frame_capture")
+
+    def frame_replay(self):
+        """Launch the captured frame step on the current target."""
+        self._check(lib().twdic_frame_replay(self.h), "frame_replay")
 
     def precompute_target(self):
         self._check(lib().twdic_precompute_target(self.h), "precompute_target")

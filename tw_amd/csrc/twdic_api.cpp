@@ -214,6 +214,14 @@ FftccParams fftcc_params(twdic_ctx* c, const int32_t* d_poi, int n, float* d_u, 
     return p;
 }
 
+void drop_graph(twdic_ctx* c)
+{
+    if (c->graph_exec) (void)hipGraphExecDestroy(c->graph_exec);
+    if (c->graph) (void)hipGraphDestroy(c->graph);
+    c->graph_exec = nullptr;
+    c->graph = nullptr;
+}
+
 }  // namespace
 
 extern "C" {
@@ -296,6 +304,7 @@ twdic_status twdic_destroy(twdic_ctx* c)
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     if (c->stream2) (void)hipStreamSynchronize(c->stream2);
+    drop_graph(c);
     void* bufs[] = {c->d_ref, c->d_tar, c->d_rx, c->d_ry, c->d_tx, c->d_ty, c->d_txy, c->d_lut,
                     c->d_fft_tw, c->d_refstate, c->d_g2};
     for (void* b : bufs)
@@ -434,6 +443,63 @@ twdic_status twdic_frame(twdic_ctx* c, const int32_t* d_poi, int n, float* d_u, 
     // later work on the context stream (the next target, a sync) waits for IC-GN
     TW_HIP(c, hipEventRecord(c->pev[chunks], c->stream2));
     TW_HIP(c, hipStreamWaitEvent(c->stream, c->pev[chunks], 0));
+    return TWDIC_OK;
+}
+
+twdic_status twdic_frame_capture(twdic_ctx* c, const int32_t* d_poi, int n, float* d_u, float* d_v, float* d_zncc,
+                                 float* d_p6, int32_t* d_iters, uint8_t* d_flags)
+{
+    if (!c || n <= 0 || !d_poi || !d_u || !d_v || !d_zncc) return fail(c, TWDIC_EINVAL, "twdic_frame_capture: bad arguments");
+    if (!c->ref_loaded || !c->tar_loaded) return fail(c, TWDIC_EINVAL, "twdic_frame_capture: images not set");
+    // reference gradients and per-POI state now, outside the graph
+    IcgnParams ip;
+    twdic_status st = icgn_prepare(c, d_poi, n, d_u, d_v, d_p6, d_iters, d_flags, nullptr, ip);
+    if (st != TWDIC_OK) return st;
+    const FftccParams fp = fftcc_params(c, d_poi, n, d_u, d_v, d_zncc, nullptr);
+    const twdic_config& g = c->cfg;
+    drop_graph(c);
+    TW_HIP(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    hipError_t e = hipSuccess;
+    if (g.interp == TWDIC_INTERP_LUT)
+        e = launch_bicubic_lut(c->d_tar, c->dstride, g.roi_x, g.roi_y, g.roi_w, g.roi_h, c->d_lut, c->stream);
+    else if (g.interp == TWDIC_INTERP_BSPLINE)
+        e = launch_bspline_lut(c->d_tar, c->dstride, g.roi_x, g.roi_y, g.roi_w, g.roi_h, c->d_lut, c->stream);
+    if (e == hipSuccess) e = launch_fftcc(fp, c->fftcc_variant, c->stream);
+    if (e == hipSuccess) e = launch_icgn(ip, c->cfg.interp, c->icgn_variant, c->stream);
+    hipGraph_t gr = nullptr;
+    const hipError_t e2 = hipStreamEndCapture(c->stream, &gr);
+    if (e != hipSuccess || e2 != hipSuccess) {
+        if (gr) (void)hipGraphDestroy(gr);
+        return hip_fail(c, e != hipSuccess ? e : e2, "twdic_frame_capture");
+    }
+    c->graph = gr;
+    e = hipGraphInstantiate(&c->graph_exec, c->graph, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+        drop_graph(c);
+        return hip_fail(c, e, "hipGraphInstantiate");
+    }
+    c->graph_ip = ip;
+    return TWDIC_OK;
+}
+
+twdic_status twdic_frame_replay(twdic_ctx* c)
+{
+    if (!c) return TWDIC_EINVAL;
+    if (!c->graph_exec) return fail(c, TWDIC_EINVAL, "twdic_frame_replay: no captured frame");
+    if (!c->ref_loaded || !c->tar_loaded) return fail(c, TWDIC_EINVAL, "twdic_frame_replay: images not set");
+    if (c->d_refstate != c->graph_ip.refstate)
+        return fail(c, TWDIC_EINVAL, "twdic_frame_replay: the per-POI state was reallocated; capture again");
+    const IcgnParams& ip = c->graph_ip;
+    if (c->refstate_version != c->ref_version || c->refstate_poi != ip.poi || c->refstate_n != ip.n) {
+        twdic_status st = ensure_ref_grads(c);
+        if (st != TWDIC_OK) return st;
+        TW_HIP(c, launch_ref_state(ip, c->stream));
+        c->refstate_version = c->ref_version;
+        c->refstate_poi = ip.poi;
+        c->refstate_n = ip.n;
+    }
+    TW_HIP(c, hipGraphLaunch(c->graph_exec, c->stream));
+    if (c->cfg.interp != TWDIC_INTERP_FUSED) c->tar_pre_valid = true;   // the graph rebuilt the LUT
     return TWDIC_OK;
 }
 

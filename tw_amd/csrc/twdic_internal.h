@@ -176,5 +176,10 @@ struct twdic_ctx {
     hipStream_t stream2 = nullptr;
     hipEvent_t pev[kMaxFrameChunks + 1] = {};
     int frame_chunks = 1;             // TWDIC_FRAME_CHUNKS; 1 runs the two stages back to back
+    // captured frame step (twdic_frame_capture): its graph, the IC-GN
+    // parameters it was recorded with, and the per-POI state buffer they name
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
+    twdic::IcgnParams graph_ip{};
     std::string err;
 };
