@@ -213,6 +213,19 @@ def batch(a):
         bt.record(5)
         bt.sync()
         icgn_ms = bt.elapsed_ms(4, 5)
+    # the plain per-frame form on the same work: each pair alone, one after
+    # another (its FFT-CC and IC-GN launches on one stream), state built first
+    plain_ms = 0.0
+    for p in pairs:
+        with api.Batch() as b1:
+            b1.set_pairs([p])
+            b1.run()
+            b1.sync()
+            b1.record(0)
+            b1.run()
+            b1.record(1)
+            b1.sync()
+            plain_ms += b1.elapsed_ms(0, 1)
     fl = np.concatenate([b["fl"].download() for b in keep])
     out = {
         "metric": "batched form: POIs/s over 64 mixed pairs (FFT-CC + IC-GN)",
@@ -223,6 +236,9 @@ def batch(a):
                    "execution": "direct form (per-pair single-context kernels)"},
         "icgn_pois_per_s": n_tot / (icgn_ms * 1e-3),
         "latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99))},
+        "plain_form": {"ms_per_step": plain_ms, "pois_per_s": n_tot / (plain_ms * 1e-3),
+                       "batched_over_plain": plain_ms / ms,
+                       "note": "the same pairs run one at a time, each pair's FFT-CC then IC-GN on one stream"},
         "flags": np.bincount(fl, minlength=5).tolist(), "synth_s": t_gen,
     }
     alloc.close()
