@@ -156,7 +156,8 @@ def batch(a):
     {4, 5, 10}; POIs on the grid from a border of max(N/2, S + 2 +
     ceil(8 + size/100)) px (the largest displacement the field reaches).  One
     step is FFT-CC then IC-GN over every POI of every pair, through the
-    batched API (direct form: per pair, the single-context kernels)."""
+    batched API (flat form: one launch per stage and kernel form over all
+    pairs; TWDIC_BATCH_FORM=direct runs the single-context kernels per pair)."""
     from tw_amd import api
     rng = np.random.default_rng(1003)
     t_gen = time.perf_counter()
@@ -214,18 +215,27 @@ def batch(a):
         bt.sync()
         icgn_ms = bt.elapsed_ms(4, 5)
     # the plain per-frame form on the same work: each pair alone, one after
-    # another (its FFT-CC and IC-GN launches on one stream), state built first
+    # another, through the single-context kernels (a one-pair batch in the
+    # direct form), state built first
     plain_ms = 0.0
-    for p in pairs:
-        with api.Batch() as b1:
-            b1.set_pairs([p])
-            b1.run()
-            b1.sync()
-            b1.record(0)
-            b1.run()
-            b1.record(1)
-            b1.sync()
-            plain_ms += b1.elapsed_ms(0, 1)
+    form = os.environ.get("TWDIC_BATCH_FORM")
+    os.environ["TWDIC_BATCH_FORM"] = "direct"
+    try:
+        for p in pairs:
+            with api.Batch() as b1:
+                b1.set_pairs([p])
+                b1.run()
+                b1.sync()
+                b1.record(0)
+                b1.run()
+                b1.record(1)
+                b1.sync()
+                plain_ms += b1.elapsed_ms(0, 1)
+    finally:
+        if form is None:
+            del os.environ["TWDIC_BATCH_FORM"]
+        else:
+            os.environ["TWDIC_BATCH_FORM"] = form
     fl = np.concatenate([b["fl"].download() for b in keep])
     out = {
         "metric": "batched form: POIs/s over 64 mixed pairs (FFT-CC + IC-GN)",
@@ -233,12 +243,14 @@ def batch(a):
         "warmup": a.warmup_batch, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "fp32", "data": "synthetic",
         "config": {"workload": "batched64_mixed", "pairs": a.pairs, "pois": n_tot, "classes": classes,
-                   "execution": "direct form (per-pair single-context kernels)"},
+                   "execution": ("direct form (per-pair single-context kernels on four streams)"
+                                 if os.environ.get("TWDIC_BATCH_FORM") == "direct" else
+                                 "flat form (one launch per stage and kernel form over all pairs)")},
         "icgn_pois_per_s": n_tot / (icgn_ms * 1e-3),
         "latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99))},
         "plain_form": {"ms_per_step": plain_ms, "pois_per_s": n_tot / (plain_ms * 1e-3),
                        "batched_over_plain": plain_ms / ms,
-                       "note": "the same pairs run one at a time, each pair's FFT-CC then IC-GN on one stream"},
+                       "note": "the same pairs run one at a time through the single-context kernels"},
         "flags": np.bincount(fl, minlength=5).tolist(), "synth_s": t_gen,
     }
     alloc.close()

@@ -3,8 +3,12 @@
  * uneven sizes, mixed subset radii and FFT-CC windows behind one handle, with
  * resident per-pair reference state.
  *
- * Two executions.  The default build runs the direct form: per pair, the
- * single-context kernels in pair order on the batch stream.  The persistent
+ * Executions.  The default is the flat form: one launch per stage and
+ * matrix-core kernel form over the POIs of every pair, each block running the
+ * single-context per-POI code for its (pair, POI) item.
+ * TWDIC_BATCH_FORM=direct selects the direct form: per pair, the
+ * single-context kernels, round-robin on four streams.  Both give results
+ * bit-identical to the single-context API.  The persistent
  * form (kernels that drain device work queues, one launch per stage and window
  * class, hipGraph replay, launch parameters chosen by an autotuner) is compiled
  * only with -DTWDIC_BATCH_EXEC: its first run on an MI355X stalled and the
@@ -30,9 +34,9 @@ typedef struct twdic_batch twdic_batch;
 
 /* One image pair.  Images, POIs and outputs are device buffers owned by the
  * caller (borrowed for the lifetime of the batch).  Subsets are
- * (2 sy + 1) x (2 sx + 1); fft_n is any even value up to 64 (the direct form
- * runs every window size on the int8 matrix cores, as the single-context API
- * does).  Any output may be NULL except
+ * (2 sy + 1) x (2 sx + 1); fft_n is any even value up to 64 (the flat and
+ * direct forms run every window size on the int8 matrix cores, as the
+ * single-context API does).  Any output may be NULL except
  * d_u/d_v.  IC-GN interpolation is the fused bicubic evaluation. */
 typedef struct {
     const uint8_t* d_ref;
@@ -86,8 +90,9 @@ twdic_status twdic_batch_set_references(twdic_batch* b, const uint8_t* const* d_
 
 /* One batched frame step, asynchronous on the batch stream: reference
  * gradients and per-POI state when stale, then FFT-CC over every POI, then
- * IC-GN over every POI from the FFT-CC guesses (direct form: one launch per
- * pair and stage; persistent form: one launch per window class and stage). */
+ * IC-GN over every POI from the FFT-CC guesses (flat form: one launch per
+ * stage and kernel form; direct form: one launch per pair and stage;
+ * persistent form: one launch per window class and stage). */
 twdic_status twdic_batch_run(twdic_batch* b);
 /* IC-GN only, from the guesses already in d_u/d_v. */
 twdic_status twdic_batch_run_icgn(twdic_batch* b);

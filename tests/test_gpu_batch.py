@@ -19,6 +19,7 @@ SPECS = [  # (w, h, S, grid, N, deformation seed)
     (300, 200, 12, 6, 24, 4),
     (448, 448, 20, 13, 64, 5),
     (512, 480, 30, 14, 64, 6),
+    (320, 288, 18, 8, 56, 7),
 ]
 
 

@@ -1,9 +1,12 @@
-"""Batched form, direct execution (the default build): the pairs of one batch
-run the single-context kernels pair by pair, so every per-POI result must be
+"""Batched form in its two executions: flat (the default: one launch per
+stage and kernel form over the items of all pairs) and direct
+(TWDIC_BATCH_FORM=direct: the single-context kernels pair by pair).  Both run
+the single-context per-POI code, so every per-POI result must be
 bit-identical to a single context on the same pair, alone or in any mix,
 across new targets, new references and repeated runs.  The pairs mix image
-sizes, subset radii and FFT-CC windows (32, 64 and the reference's own 2S
-sizes 24 and 40)."""
+sizes, subset radii (the largest beyond the IC-GN step table) and FFT-CC
+windows (32, 64 and the reference's own 2S sizes 24 and 40, which cover three
+of the four matrix-core kernel forms)."""
 import numpy as np
 import pytest
 
@@ -18,7 +21,14 @@ def data():
     return [_make(s) for s in SPECS]
 
 
-def test_direct_batch_matches_single_context_bitwise(data):
+@pytest.fixture(params=["flat", "direct"])
+def form(request, monkeypatch):
+    monkeypatch.setenv("TWDIC_BATCH_FORM", request.param)
+    return request.param
+
+
+def test_batch_matches_single_context_bitwise(data, form):
+    print(f"[gpu-test] batch form {form}", flush=True)
     singles = [_single(p) for p in data]
     bufs = _Buffers(data)
     with api.Batch() as bt:
@@ -36,7 +46,8 @@ def test_direct_batch_matches_single_context_bitwise(data):
     bufs.alloc.close()
 
 
-def test_direct_alone_equals_in_mix_new_targets_and_references(data):
+def test_alone_equals_in_mix_new_targets_and_references(data, form):
+    print(f"[gpu-test] batch form {form}", flush=True)
     bufs = _Buffers(data)
     with api.Batch() as bt:
         bt.set_pairs(bufs.pairs(data, [2]))

@@ -1,10 +1,13 @@
 // batch_api.cpp -- the batched form (include/twdic_batch.h): descriptors and
-// work lists for many image pairs, persistent-kernel stages, hipGraph replay
-// of the per-frame sequence, and the launch-parameter autotuner.
+// work lists for many image pairs; the flat form (one launch per stage and
+// kernel form over all pairs, the default), the direct form (per pair, the
+// single-context kernels on four streams), and the persistent-kernel stages
+// with hipGraph replay and the launch-parameter autotuner (switched off).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
 #include <cmath>
+#include <cstdint>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -53,6 +56,12 @@ struct twdic_batch {
     hipStream_t lane[kLanes] = {};
     hipEvent_t fork = nullptr, join[kLanes] = {};
     int lanes = 4;                    // TWDIC_BATCH_STREAMS (1..4)
+    // flat form: one launch per stage and kernel form over all pairs
+    bool flat = true;                 // TWDIC_BATCH_FORM=direct selects the direct form
+    int2* d_mitems[4] = {};           // FFT-CC items (pair, POI) per matrix-core kernel form (mma_form)
+    int64_t nm[4] = {};
+    int2* d_wg[2] = {};               // IC-GN workgroup items (pair, first POI): step table fits, does not
+    int64_t nwg[2] = {};
 };
 
 namespace {
@@ -74,14 +83,16 @@ int fclass(int N) { return N == 64 ? 0 : N == 32 ? 1 : 2; }
 // The persistent-kernel form (work queues, hipGraph replay, autotuner)
 // stalled on its first run on an MI355X and the cause has not been found; it
 // is compiled only with -DTWDIC_BATCH_EXEC.  Without it, a batch runs in the
-// direct form: per pair, the single-context kernels (launch_fftcc,
-// launch_icgn, launch_ref_state) in pair order on the batch stream.
+// flat form (one launch per stage and kernel form over the items of all
+// pairs) or, with TWDIC_BATCH_FORM=direct, in the direct form (per pair, the
+// single-context kernels launch_fftcc and launch_icgn on four streams).  The
+// per-POI reference state is built per pair (launch_ref_state) in both.
 #ifdef TWDIC_BATCH_EXEC
 #define TB_EXEC_GATE(b) (void)0
 constexpr bool kPersistent = true;
 #else
 #define TB_EXEC_GATE(b) \
-    return bfail((b), TWDIC_EINVAL, "the persistent batched form is switched off in this build (stalled on its first run)")
+    return bfail((b), TWDIC_EINVAL, "the persistent batched form is compiled only with TWDIC_BATCH_EXEC (it stalled on its first run)")
 constexpr bool kPersistent = false;
 #endif
 
@@ -93,9 +104,12 @@ void release(twdic_batch* b)
     b->graph_exec = nullptr;
     b->graph = nullptr;
     void* bufs[] = {b->d_desc, b->d_items, b->d_fitems[0], b->d_fitems[1], b->d_fitems[2], b->d_g2, b->d_state,
-                    b->d_zncc_scratch, b->d_counters};
+                    b->d_zncc_scratch, b->d_counters, b->d_mitems[0], b->d_mitems[1], b->d_mitems[2],
+                    b->d_mitems[3], b->d_wg[0], b->d_wg[1]};
     for (void* p : bufs)
         if (p) (void)hipFree(p);
+    for (int f = 0; f < 4; ++f) { b->d_mitems[f] = nullptr; b->nm[f] = 0; }
+    for (int t = 0; t < 2; ++t) { b->d_wg[t] = nullptr; b->nwg[t] = 0; }
     for (auto& kv : b->tw) (void)hipFree(kv.second);
     b->tw.clear();
     b->d_desc = nullptr; b->d_items = nullptr;
@@ -199,6 +213,24 @@ twdic_status run_direct(twdic_batch* b, bool fftcc, bool icgn)
     return TWDIC_OK;
 }
 
+// Flat form: FFT-CC per matrix-core kernel form, then IC-GN per step-table
+// fit, each one launch over the items of every pair on the batch stream.  The
+// same per-POI device code as the single-context kernels.
+twdic_status run_flat(twdic_batch* b, bool fftcc, bool icgn)
+{
+    if (!b->state_valid) {
+        twdic_status st = build_state_direct(b);
+        if (st != TWDIC_OK) return st;
+    }
+    if (fftcc)
+        for (int f = 0; f < 4; ++f)
+            if (b->nm[f]) TB_HIP(b, launch_batch_fftcc_mma(b->d_desc, b->d_mitems[f], (int)b->nm[f], f, b->stream));
+    if (icgn)
+        for (int t = 0; t < 2; ++t)
+            if (b->nwg[t]) TB_HIP(b, launch_batch_icgn_wg(b->d_desc, b->d_wg[t], (int)b->nwg[t], t == 0, b->stream));
+    return TWDIC_OK;
+}
+
 std::string signature(twdic_batch* b)
 {
     hipDeviceProp_t prop;
@@ -242,6 +274,7 @@ twdic_status twdic_batch_create(int device, twdic_batch** out)
         return TWDIC_EHIP;
     }
     if (const char* v = std::getenv("TWDIC_BATCH_STREAMS")) b->lanes = std::atoi(v);
+    if (const char* v = std::getenv("TWDIC_BATCH_FORM")) b->flat = std::strcmp(v, "direct") != 0;
     *out = b;
     return TWDIC_OK;
 }
@@ -337,6 +370,28 @@ twdic_status twdic_batch_set_pairs(twdic_batch* b, const twdic_pair* pairs, int 
         }
     }
     if (total) TB_HIP(b, hipMemcpy(b->d_items, items.data(), sizeof(int2) * total, hipMemcpyHostToDevice));
+    // flat form: FFT-CC items per kernel form, IC-GN items of four POIs of one pair
+    std::vector<int2> mitems[4], wg[2];
+    for (int i = 0; i < npairs; ++i) {
+        const twdic_pair& p = b->pairs[i];
+        auto& mi = mitems[mma_form(p.fft_n)];
+        for (int k = 0; k < p.n; ++k) mi.push_back(make_int2(i, k));
+        auto& w = wg[step_table_fits((2 * p.sx + 1) * (2 * p.sy + 1)) ? 0 : 1];
+        for (int k = 0; k < p.n; k += 4) w.push_back(make_int2(i, k));
+    }
+    for (int f = 0; f < 4; ++f) {
+        b->nm[f] = (int64_t)mitems[f].size();
+        if (b->nm[f] > INT32_MAX) return bfail(b, TWDIC_EINVAL, "more than 2^31 - 1 POIs in one FFT-CC kernel form");
+        if (!b->nm[f]) continue;
+        TB_HIP(b, alloc((void**)&b->d_mitems[f], sizeof(int2) * mitems[f].size()));
+        TB_HIP(b, hipMemcpy(b->d_mitems[f], mitems[f].data(), sizeof(int2) * mitems[f].size(), hipMemcpyHostToDevice));
+    }
+    for (int t = 0; t < 2; ++t) {
+        b->nwg[t] = (int64_t)wg[t].size();
+        if (!b->nwg[t]) continue;
+        TB_HIP(b, alloc((void**)&b->d_wg[t], sizeof(int2) * wg[t].size()));
+        TB_HIP(b, hipMemcpy(b->d_wg[t], wg[t].data(), sizeof(int2) * wg[t].size(), hipMemcpyHostToDevice));
+    }
     for (int c = 0; c < 3; ++c) {
         b->nf[c] = (int64_t)fitems[c].size();
         if (!b->nf[c]) continue;
@@ -378,7 +433,7 @@ twdic_status twdic_batch_run(twdic_batch* b)
     if (!b) return TWDIC_EINVAL;
     if (!b->total) return TWDIC_OK;
     (void)hipSetDevice(b->device);
-    if (!kPersistent) return run_direct(b, true, true);
+    if (!kPersistent) return b->flat ? run_flat(b, true, true) : run_direct(b, true, true);
     if (!b->state_valid) {
         twdic_status st = build_state(b);
         if (st != TWDIC_OK) return st;
@@ -411,7 +466,7 @@ twdic_status twdic_batch_run_icgn(twdic_batch* b)
     if (!b) return TWDIC_EINVAL;
     if (!b->total) return TWDIC_OK;
     (void)hipSetDevice(b->device);
-    if (!kPersistent) return run_direct(b, false, true);
+    if (!kPersistent) return b->flat ? run_flat(b, false, true) : run_direct(b, false, true);
     if (!b->state_valid) {
         twdic_status st = build_state(b);
         if (st != TWDIC_OK) return st;

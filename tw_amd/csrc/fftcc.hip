@@ -852,12 +852,11 @@ __device__ __forceinline__ void load_row(const uint8_t* src, int n, uint32_t (&w
 // the whole 64-byte row as one K chunk, for 32 < N <= 48, where it issues
 // 9 16-cycle MFMAs per row instead of 8 32-cycle ones.
 template <int NT, bool N64, int TILE>
-__global__ __launch_bounds__(64) void k_fftcc_mma(FftccParams p)
+__device__ __forceinline__ void fftcc_mma_poi(typename MmaSel<NT, N64, TILE>::type& sm, const FftccParams& p,
+                                              int poi)
 {
     constexpr int TT = NT / 32;          // output tiles per edge and K chunks per row (TILE 32)
     using Sm = typename MmaSel<NT, N64, TILE>::type;
-    __shared__ Sm sm;
-    const int poi = blockIdx.x;
     const int lane = threadIdx.x, m = lane & 31, h = lane >> 5;
     const int N = N64 ? 64 : p.N, NN = N * N;
     const int py = p.poi[2 * poi], px = p.poi[2 * poi + 1];
@@ -1103,7 +1102,43 @@ __global__ __launch_bounds__(64) void k_fftcc_mma(FftccParams p)
     }
 }
 
+template <int NT, bool N64, int TILE>
+__global__ __launch_bounds__(64) void k_fftcc_mma(FftccParams p)
+{
+    __shared__ typename MmaSel<NT, N64, TILE>::type sm;
+    fftcc_mma_poi<NT, N64, TILE>(sm, p, blockIdx.x);
+}
+
+// Batched form, one launch over the POIs of many image pairs that share a
+// kernel form: block b takes item b, (pair, POI), and that pair's descriptor.
+template <int NT, bool N64, int TILE>
+__global__ __launch_bounds__(64) void k_batch_fftcc_mma(const PairDesc* __restrict__ pairs,
+                                                        const int2* __restrict__ items)
+{
+    __shared__ typename MmaSel<NT, N64, TILE>::type sm;
+    const int2 it = items[blockIdx.x];
+    const FftccParams p = const_pairs(pairs)[__builtin_amdgcn_readfirstlane(it.x)].fftcc;
+    fftcc_mma_poi<NT, N64, TILE>(sm, p, __builtin_amdgcn_readfirstlane(it.y));
+}
+
 }  // namespace
+
+int mma_form(int N)
+{
+    return N == 64 ? 0 : N > 48 ? 1 : N > 32 ? 2 : 3;
+}
+
+hipError_t launch_batch_fftcc_mma(const PairDesc* pairs, const int2* items, int count, int form, hipStream_t st)
+{
+    if (count <= 0) return hipSuccess;
+    switch (form) {
+    case 0: hipLaunchKernelGGL((k_batch_fftcc_mma<64, true, 32>), dim3(count), dim3(64), 0, st, pairs, items); break;
+    case 1: hipLaunchKernelGGL((k_batch_fftcc_mma<64, false, 32>), dim3(count), dim3(64), 0, st, pairs, items); break;
+    case 2: hipLaunchKernelGGL((k_batch_fftcc_mma<64, false, 16>), dim3(count), dim3(64), 0, st, pairs, items); break;
+    default: hipLaunchKernelGGL((k_batch_fftcc_mma<32, false, 32>), dim3(count), dim3(64), 0, st, pairs, items); break;
+    }
+    return hipGetLastError();
+}
 
 // variant: occupancy target of the register engine (waves per SIMD the
 // compiler budgets registers for): 2 (no spills), 3 or 4.

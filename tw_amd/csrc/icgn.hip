@@ -415,6 +415,45 @@ __global__ __launch_bounds__(256, MINW > 0 ? MINW : 1) void k_icgn(IcgnParams P)
     }
 }
 
+// Batched form, one launch over many image pairs: workgroup item w =
+// (pair, first POI) holds up to four POIs of one pair, so the workgroup's step
+// table is that pair's.  Items go to the XCDs round-robin, in block order:
+// work per item differs from pair to pair (subset size, iteration counts),
+// and contiguous per-XCD ranges as in k_icgn would leave XCDs idle.
+template <int INTERP, int MINW, bool TABLE>
+__global__ __launch_bounds__(256, MINW > 0 ? MINW : 1) void k_batch_icgn_wg(const PairDesc* __restrict__ pairs,
+                                                                           const int2* __restrict__ wg)
+{
+    const int2 it = wg[blockIdx.x];
+    const IcgnParams P = const_pairs(pairs)[__builtin_amdgcn_readfirstlane(it.x)].icgn;
+    const int poi = __builtin_amdgcn_readfirstlane(it.y + (int)(threadIdx.x >> 6));
+    if constexpr (TABLE) {
+        __shared__ StepTable tab;
+        fill_step_table(tab, P);
+        __syncthreads();
+        if (poi >= P.n) return;
+        icgn_poi<INTERP, true>(P, poi, threadIdx.x & 63, &tab);
+    } else {
+        if (poi >= P.n) return;
+        icgn_poi<INTERP, false>(P, poi, threadIdx.x & 63, nullptr);
+    }
+}
+
+bool step_table_fits(int s)
+{
+    return s <= kStepTable;
+}
+
+hipError_t launch_batch_icgn_wg(const PairDesc* pairs, const int2* wg, int count, bool table, hipStream_t st)
+{
+    if (count <= 0) return hipSuccess;
+    if (table)
+        hipLaunchKernelGGL((k_batch_icgn_wg<TWDIC_INTERP_FUSED, 5, true>), dim3(count), dim3(256), 0, st, pairs, wg);
+    else
+        hipLaunchKernelGGL((k_batch_icgn_wg<TWDIC_INTERP_FUSED, 5, false>), dim3(count), dim3(256), 0, st, pairs, wg);
+    return hipGetLastError();
+}
+
 // Persistent form: every wavefront takes `chunk` consecutive work items
 // (image pair, POI) from a device counter until the list is exhausted; each
 // item's geometry and buffers come from its pair's descriptor.  Every wave

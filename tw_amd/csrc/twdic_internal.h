@@ -115,6 +115,16 @@ hipError_t launch_poi_grid(int32_t* poi, int start_x, int start_y, int nx, int n
 // else the direct-DFT engine)
 constexpr int kFftccMma = 8;
 hipError_t launch_fftcc(const FftccParams& p, int variant, hipStream_t st);
+// batched form: the matrix-core kernel over items (pair, POI) of pairs that
+// share a kernel form (mma_form of their window edge N: 0 for N = 64,
+// 1 for 48 < N < 64, 2 for 32 < N <= 48, 3 for N <= 32)
+int mma_form(int N);
+hipError_t launch_batch_fftcc_mma(const PairDesc* pairs, const int2* items, int count, int form, hipStream_t st);
+// batched form: IC-GN (fused interpolation) over workgroup items (pair, first
+// POI), four POIs of one pair per workgroup; `table` when every listed pair's
+// subset fits the step table (step_table_fits)
+bool step_table_fits(int s);
+hipError_t launch_batch_icgn_wg(const PairDesc* pairs, const int2* wg, int count, bool table, hipStream_t st);
 hipError_t launch_ref_state(const IcgnParams& p, hipStream_t st);
 hipError_t launch_icgn(const IcgnParams& p, int interp, int variant, hipStream_t st);
 // persistent batch kernels: `blocks` workgroups drain a device work counter
