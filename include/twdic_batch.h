@@ -12,8 +12,10 @@
  * form (kernels that drain device work queues, one launch per stage and window
  * class, hipGraph replay, launch parameters chosen by an autotuner) is compiled
  * only with -DTWDIC_BATCH_EXEC: its first run on an MI355X stalled and the
- * cause has not been found.  In the default build the tuning calls are
- * accepted and have no effect, and twdic_batch_autotune returns TWDIC_EINVAL.
+ * cause has not been found.  In the default build twdic_batch_autotune
+ * returns TWDIC_EINVAL; of the tuning fields only use_graph has an effect:
+ * the flat form then replays its whole step (FFT-CC and IC-GN) from a
+ * hipGraph, captured on the first run after new pairs or new tuning.
  *
  * No counterpart in the reference, which runs one image pair per
  * cuFFTCC2D/cuICGN2D object and one CUDA block per POI
@@ -62,7 +64,7 @@ typedef struct {
 typedef struct {
     int icgn_chunk;     /* POIs per IC-GN work-queue grab (1..64) */
     int fftcc_variant;  /* occupancy target (waves/SIMD) of the N = 64 engine: 2, 3 or 4 */
-    int use_graph;      /* 1: replay the per-frame stage sequence as a hipGraph */
+    int use_graph;      /* 1: replay the per-frame stage sequence as a hipGraph (flat and persistent forms) */
 } twdic_batch_tuning;
 
 typedef struct {

@@ -83,3 +83,36 @@ def test_persistent_form_calls_are_refused():
     with api.Batch() as bt:
         with pytest.raises(api.TwdicError):
             bt.autotune(None, reps=1)
+
+
+def test_flat_graph_replays_equal_eager_steps(data, monkeypatch):
+    """The flat form's step replayed from its hipGraph (the default tuning)
+    equals the eagerly launched step, bit for bit, on every replay: three
+    steps over alternating targets, with eager single-context work and syncs
+    between them."""
+    monkeypatch.setenv("TWDIC_BATCH_FORM", "flat")
+    print("[gpu-test] flat form, graph replay against eager steps", flush=True)
+    w, h, S, g, N, seed = SPECS[0]
+    _, t2 = api.synth_pair(w, h, int(0.015 * w * h), 700 + seed, api.affine(1.5, -0.75, 0, 0.003, 0, w / 2, h / 2),
+                           want_ref=False)
+    runs = {}
+    for graph in (0, 1):
+        bufs = _Buffers(data)
+        dt2 = bufs.alloc.array((h, w), np.uint8).upload(t2)
+        bufs.alloc.sync()
+        out = []
+        with api.Batch() as bt:
+            bt.set_tuning(use_graph=graph)
+            bt.set_pairs(bufs.pairs(data))
+            for k in range(3):
+                tar0 = dt2.ptr if k == 1 else bufs.bufs[0]["tar"].ptr
+                bt.set_targets([tar0] + [None] * (len(data) - 1))
+                bt.run()
+                bt.sync()
+                out.append([bufs.results(i) for i in range(len(data))])
+                _single(data[k])                    # eager work between the steps
+        runs[graph] = out
+        bufs.alloc.close()
+    for k in range(3):
+        for i in range(len(data)):
+            _same(runs[1][k][i], runs[0][k][i])

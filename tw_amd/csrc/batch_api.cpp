@@ -216,18 +216,47 @@ twdic_status run_direct(twdic_batch* b, bool fftcc, bool icgn)
 // Flat form: FFT-CC per matrix-core kernel form, then IC-GN per step-table
 // fit, each one launch over the items of every pair on the batch stream.  The
 // same per-POI device code as the single-context kernels.
-twdic_status run_flat(twdic_batch* b, bool fftcc, bool icgn)
+twdic_status enqueue_flat(twdic_batch* b, bool fftcc, bool icgn)
 {
-    if (!b->state_valid) {
-        twdic_status st = build_state_direct(b);
-        if (st != TWDIC_OK) return st;
-    }
     if (fftcc)
         for (int f = 0; f < 4; ++f)
             if (b->nm[f]) TB_HIP(b, launch_batch_fftcc_mma(b->d_desc, b->d_mitems[f], (int)b->nm[f], f, b->stream));
     if (icgn)
         for (int t = 0; t < 2; ++t)
             if (b->nwg[t]) TB_HIP(b, launch_batch_icgn_wg(b->d_desc, b->d_wg[t], (int)b->nwg[t], t == 0, b->stream));
+    return TWDIC_OK;
+}
+
+// The full step (FFT-CC and IC-GN) is replayed from a hipGraph when the
+// tuning asks for it.  Its kernels read the pair descriptors from device
+// memory, so new targets and references need no new capture; new pairs (item
+// lists) or tuning mark it stale.  The per-POI reference state is rebuilt
+// eagerly, before the launch, when a reference changed.
+twdic_status run_flat(twdic_batch* b, bool fftcc, bool icgn)
+{
+    if (!b->state_valid) {
+        twdic_status st = build_state_direct(b);
+        if (st != TWDIC_OK) return st;
+    }
+    if (!(fftcc && icgn && b->tune.use_graph)) return enqueue_flat(b, fftcc, icgn);
+    if (b->graph_dirty) {
+        if (b->graph_exec) (void)hipGraphExecDestroy(b->graph_exec);
+        if (b->graph) (void)hipGraphDestroy(b->graph);
+        b->graph_exec = nullptr;
+        b->graph = nullptr;
+        TB_HIP(b, hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+        const twdic_status st = enqueue_flat(b, true, true);
+        hipGraph_t g = nullptr;
+        const hipError_t e = hipStreamEndCapture(b->stream, &g);
+        if (st != TWDIC_OK || e != hipSuccess) {
+            if (g) (void)hipGraphDestroy(g);
+            return st != TWDIC_OK ? st : bfail(b, TWDIC_EHIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+        }
+        b->graph = g;
+        TB_HIP(b, hipGraphInstantiate(&b->graph_exec, b->graph, nullptr, nullptr, 0));
+        b->graph_dirty = false;
+    }
+    TB_HIP(b, hipGraphLaunch(b->graph_exec, b->stream));
     return TWDIC_OK;
 }
 
