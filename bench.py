@@ -189,10 +189,15 @@ def batch(a):
             print(f"batch: {i + 1} pairs synthesised, {n_tot} POIs", file=sys.stderr, flush=True)
     alloc.sync()
     t_gen = time.perf_counter() - t_gen
+    tuned = None
     with api.Batch() as bt:
         bt.set_pairs(pairs)
         bt.run()                                   # reference state + first frame
         bt.sync()
+        if os.environ.get("TWDIC_BATCH_FORM") != "direct":
+            bt.autotune(None, reps=2)              # flat form: IC-GN occupancy and XCD order
+            t = bt.tuning
+            tuned = {"icgn_waves": t.icgn_waves, "icgn_xcd_ranges": t.icgn_xcd_ranges, "use_graph": t.use_graph}
         for _ in range(a.warmup_batch):
             bt.run()
         bt.sync()
@@ -248,6 +253,7 @@ def batch(a):
                                  "flat form (one launch per stage and kernel form over all pairs)")},
         "icgn_pois_per_s": n_tot / (icgn_ms * 1e-3),
         "latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99))},
+        "tuning": tuned,
         "plain_form": {"ms_per_step": plain_ms, "pois_per_s": n_tot / (plain_ms * 1e-3),
                        "batched_over_plain": plain_ms / ms,
                        "note": "the same pairs run one at a time through the single-context kernels"},

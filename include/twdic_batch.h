@@ -12,10 +12,11 @@
  * form (kernels that drain device work queues, one launch per stage and window
  * class, hipGraph replay, launch parameters chosen by an autotuner) is compiled
  * only with -DTWDIC_BATCH_EXEC: its first run on an MI355X stalled and the
- * cause has not been found.  In the default build twdic_batch_autotune
- * returns TWDIC_EINVAL; of the tuning fields only use_graph has an effect:
- * the flat form then replays its whole step (FFT-CC and IC-GN) from a
- * hipGraph, captured on the first run after new pairs or new tuning.
+ * cause has not been found.  In the default build the flat form reads
+ * use_graph (replay of its whole step, FFT-CC and IC-GN, from a hipGraph
+ * captured on the first run after new pairs or new tuning), icgn_waves and
+ * icgn_xcd_ranges; twdic_batch_autotune tunes those two for the flat form and
+ * returns TWDIC_EINVAL for the direct form.
  *
  * No counterpart in the reference, which runs one image pair per
  * cuFFTCC2D/cuICGN2D object and one CUDA block per POI
@@ -65,6 +66,9 @@ typedef struct {
     int icgn_chunk;     /* POIs per IC-GN work-queue grab (1..64) */
     int fftcc_variant;  /* occupancy target (waves/SIMD) of the N = 64 engine: 2, 3 or 4 */
     int use_graph;      /* 1: replay the per-frame stage sequence as a hipGraph (flat and persistent forms) */
+    int icgn_waves;     /* flat form: occupancy target (waves/SIMD) of the IC-GN kernel: 5 or 6 */
+    int icgn_xcd_ranges;/* flat form: 1 gives each XCD a contiguous range of IC-GN workgroup items,
+                           0 deals them out in block order */
 } twdic_batch_tuning;
 
 typedef struct {

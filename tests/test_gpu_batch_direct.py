@@ -79,10 +79,41 @@ def test_alone_equals_in_mix_new_targets_and_references(data, form):
     bufs.alloc.close()
 
 
-def test_persistent_form_calls_are_refused():
+def test_direct_form_refuses_autotune(monkeypatch):
+    monkeypatch.setenv("TWDIC_BATCH_FORM", "direct")
     with api.Batch() as bt:
         with pytest.raises(api.TwdicError):
             bt.autotune(None, reps=1)
+
+
+def test_flat_autotune_picks_a_candidate_and_keeps_results(data, monkeypatch, tmp_path):
+    """The flat form's autotuner times the four IC-GN configurations (5 or 6
+    waves per SIMD, items dealt out or in per-XCD ranges), keeps one of them,
+    and writes it to the table, which a new batch on the same pairs loads;
+    results after tuning are still bit-identical to single contexts."""
+    monkeypatch.setenv("TWDIC_BATCH_FORM", "flat")
+    print("[gpu-test] flat form autotune", flush=True)
+    singles = [_single(p) for p in data]
+    bufs = _Buffers(data)
+    table = tmp_path / "tuning.tsv"
+    with api.Batch() as bt:
+        bt.set_pairs(bufs.pairs(data))
+        bt.autotune(str(table), reps=1)
+        t = bt.tuning
+        assert t.icgn_waves in (5, 6) and t.icgn_xcd_ranges in (0, 1)
+        bt.run()
+        bt.sync()
+        for i in range(len(data)):
+            _same(bufs.results(i), singles[i])
+    line = table.read_text().strip().splitlines()[-1]
+    fields = line.split("\t")[1].split()
+    assert [int(x) for x in fields[3:5]] == [t.icgn_waves, t.icgn_xcd_ranges]
+    with api.Batch() as bt2:
+        bt2.set_pairs(bufs.pairs(data))
+        assert bt2.load_tuning(str(table))
+        t2 = bt2.tuning
+        assert (t2.icgn_waves, t2.icgn_xcd_ranges) == (t.icgn_waves, t.icgn_xcd_ranges)
+    bufs.alloc.close()
 
 
 def test_flat_graph_replays_equal_eager_steps(data, monkeypatch):

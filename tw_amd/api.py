@@ -54,7 +54,8 @@ class Pair(C.Structure):
 
 
 class Tuning(C.Structure):
-    _fields_ = [("icgn_chunk", C.c_int), ("fftcc_variant", C.c_int), ("use_graph", C.c_int)]
+    _fields_ = [("icgn_chunk", C.c_int), ("fftcc_variant", C.c_int), ("use_graph", C.c_int),
+                ("icgn_waves", C.c_int), ("icgn_xcd_ranges", C.c_int)]
 
 
 class ServerConfig(C.Structure):
@@ -459,8 +460,8 @@ class Batch:
         self._check(lib().twdic_batch_get_tuning(self.h, C.byref(t)), "get_tuning")
         return t
 
-    def set_tuning(self, icgn_chunk=4, fftcc_variant=3, use_graph=1):
-        t = Tuning(icgn_chunk, fftcc_variant, use_graph)
+    def set_tuning(self, icgn_chunk=4, fftcc_variant=3, use_graph=1, icgn_waves=5, icgn_xcd_ranges=0):
+        t = Tuning(icgn_chunk, fftcc_variant, use_graph, icgn_waves, icgn_xcd_ranges)
         self._check(lib().twdic_batch_set_tuning(self.h, C.byref(t)), "set_tuning")
 
     def autotune(self, table_path: str | None = None, reps: int = 3):

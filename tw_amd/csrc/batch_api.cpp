@@ -44,7 +44,7 @@ struct twdic_batch {
     std::map<int, float2*> tw;        // FFT twiddle tables per window edge
     int* d_counters = nullptr;
     bool state_valid = false;
-    twdic_batch_tuning tune{4, 3, 1};
+    twdic_batch_tuning tune{4, 3, 1, 5, 0};
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
     bool graph_dirty = true;
@@ -223,7 +223,9 @@ twdic_status enqueue_flat(twdic_batch* b, bool fftcc, bool icgn)
             if (b->nm[f]) TB_HIP(b, launch_batch_fftcc_mma(b->d_desc, b->d_mitems[f], (int)b->nm[f], f, b->stream));
     if (icgn)
         for (int t = 0; t < 2; ++t)
-            if (b->nwg[t]) TB_HIP(b, launch_batch_icgn_wg(b->d_desc, b->d_wg[t], (int)b->nwg[t], t == 0, b->stream));
+            if (b->nwg[t])
+                TB_HIP(b, launch_batch_icgn_wg(b->d_desc, b->d_wg[t], (int)b->nwg[t], t == 0, b->tune.icgn_waves,
+                                               b->tune.icgn_xcd_ranges != 0, b->stream));
     return TWDIC_OK;
 }
 
@@ -520,17 +522,32 @@ twdic_status twdic_batch_get_tuning(twdic_batch* b, twdic_batch_tuning* t)
 twdic_status twdic_batch_set_tuning(twdic_batch* b, const twdic_batch_tuning* t)
 {
     if (!b || !t) return TWDIC_EINVAL;
-    if (t->icgn_chunk < 1 || t->icgn_chunk > 64 || t->fftcc_variant < 2 || t->fftcc_variant > 4)
+    if (t->icgn_chunk < 1 || t->icgn_chunk > 64 || t->fftcc_variant < 2 || t->fftcc_variant > 4 ||
+        (t->icgn_waves != 5 && t->icgn_waves != 6) || (t->icgn_xcd_ranges != 0 && t->icgn_xcd_ranges != 1))
         return bfail(b, TWDIC_EINVAL, "tuning out of range");
     b->tune = *t;
     b->graph_dirty = true;
     return TWDIC_OK;
 }
 
+namespace {
+
+twdic_status append_tuning(twdic_batch* b, const char* table_path, const twdic_batch_tuning& t, float ms_per_step)
+{
+    if (!table_path) return TWDIC_OK;
+    std::ofstream f(table_path, std::ios::app);
+    if (!f) return bfail(b, TWDIC_EINVAL, std::string("cannot append to ") + table_path);
+    f << signature(b) << '\t' << t.icgn_chunk << ' ' << t.fftcc_variant << ' ' << t.use_graph << ' ' << t.icgn_waves
+      << ' ' << t.icgn_xcd_ranges << ' ' << ms_per_step << '\n';
+    return TWDIC_OK;
+}
+
+}  // namespace
+
 twdic_status twdic_batch_autotune(twdic_batch* b, const char* table_path, int reps)
 {
     if (!b || reps < 1) return TWDIC_EINVAL;
-    TB_EXEC_GATE(b);
+    if (!kPersistent && !b->flat) return bfail(b, TWDIC_EINVAL, "the direct batched form has nothing to tune");
     if (!b->total) return TWDIC_OK;
     auto time_cfg = [&](const twdic_batch_tuning& t, float* ms) -> twdic_status {
         twdic_status st = twdic_batch_set_tuning(b, &t);
@@ -546,6 +563,22 @@ twdic_status twdic_batch_autotune(twdic_batch* b, const char* table_path, int re
     };
     twdic_batch_tuning best = b->tune;
     float best_ms = 1e30f;
+    if (!kPersistent) {
+        // flat form: IC-GN occupancy target x workgroup-to-XCD order
+        for (int waves : {5, 6})
+            for (int xcd : {0, 1}) {
+                twdic_batch_tuning t = best;
+                t.icgn_waves = waves;
+                t.icgn_xcd_ranges = xcd;
+                float ms;
+                twdic_status st = time_cfg(t, &ms);
+                if (st != TWDIC_OK) return st;
+                if (ms < best_ms) { best_ms = ms; best = t; }
+            }
+        twdic_status st = twdic_batch_set_tuning(b, &best);
+        if (st != TWDIC_OK) return st;
+        return append_tuning(b, table_path, best, best_ms / reps);
+    }
     // stage 1: FFT-CC occupancy target (only matters with N = 64 items)
     const int variants[3] = {2, 3, 4};
     for (int v : variants) {
@@ -569,13 +602,7 @@ twdic_status twdic_batch_autotune(twdic_batch* b, const char* table_path, int re
     }
     twdic_status st = twdic_batch_set_tuning(b, &best);
     if (st != TWDIC_OK) return st;
-    if (table_path) {
-        std::ofstream f(table_path, std::ios::app);
-        if (!f) return bfail(b, TWDIC_EINVAL, std::string("cannot append to ") + table_path);
-        f << signature(b) << '\t' << best.icgn_chunk << ' ' << best.fftcc_variant << ' ' << best.use_graph << ' '
-          << best_ms / reps << '\n';
-    }
-    return TWDIC_OK;
+    return append_tuning(b, table_path, best, best_ms / reps);
 }
 
 twdic_status twdic_batch_load_tuning(twdic_batch* b, const char* table_path, int* found)
@@ -592,8 +619,11 @@ twdic_status twdic_batch_load_tuning(twdic_batch* b, const char* table_path, int
         auto tab = line.find('\t');
         if (tab == std::string::npos || line.compare(0, tab, sig) != 0 || tab != sig.size()) continue;
         std::istringstream is(line.substr(tab + 1));
-        twdic_batch_tuning c{};
+        twdic_batch_tuning c = b->tune;
         if (is >> c.icgn_chunk >> c.fftcc_variant >> c.use_graph) {
+            // the flat-form fields follow; lines without them keep the defaults
+            int w = 0, x = 0;
+            if (is >> w >> x) { c.icgn_waves = w; c.icgn_xcd_ranges = x; }
             t = c;
             hit = true;
         }

@@ -417,14 +417,14 @@ __global__ __launch_bounds__(256, MINW > 0 ? MINW : 1) void k_icgn(IcgnParams P)
 
 // Batched form, one launch over many image pairs: workgroup item w =
 // (pair, first POI) holds up to four POIs of one pair, so the workgroup's step
-// table is that pair's.  Items go to the XCDs round-robin, in block order:
-// work per item differs from pair to pair (subset size, iteration counts),
-// and contiguous per-XCD ranges as in k_icgn would leave XCDs idle.
-template <int INTERP, int MINW, bool TABLE>
+// table is that pair's.  XCD: items in contiguous per-XCD ranges as in
+// k_icgn (one band of the work per L2); otherwise dealt out in block order,
+// which balances the XCDs when the work per item differs from pair to pair.
+template <int INTERP, int MINW, bool TABLE, bool XCD>
 __global__ __launch_bounds__(256, MINW > 0 ? MINW : 1) void k_batch_icgn_wg(const PairDesc* __restrict__ pairs,
                                                                            const int2* __restrict__ wg)
 {
-    const int2 it = wg[blockIdx.x];
+    const int2 it = wg[XCD ? xcd_block(blockIdx.x, gridDim.x) : (int)blockIdx.x];
     const IcgnParams P = const_pairs(pairs)[__builtin_amdgcn_readfirstlane(it.x)].icgn;
     const int poi = __builtin_amdgcn_readfirstlane(it.y + (int)(threadIdx.x >> 6));
     if constexpr (TABLE) {
@@ -444,13 +444,29 @@ bool step_table_fits(int s)
     return s <= kStepTable;
 }
 
-hipError_t launch_batch_icgn_wg(const PairDesc* pairs, const int2* wg, int count, bool table, hipStream_t st)
+template <int MINW, bool TABLE, bool XCD>
+static void launch_wg(const PairDesc* pairs, const int2* wg, int count, hipStream_t st)
+{
+    hipLaunchKernelGGL((k_batch_icgn_wg<TWDIC_INTERP_FUSED, MINW, TABLE, XCD>), dim3(count), dim3(256), 0, st, pairs, wg);
+}
+
+template <int MINW>
+static void launch_wg_w(const PairDesc* pairs, const int2* wg, int count, bool table, bool xcd, hipStream_t st)
+{
+    if (table)
+        xcd ? launch_wg<MINW, true, true>(pairs, wg, count, st) : launch_wg<MINW, true, false>(pairs, wg, count, st);
+    else
+        xcd ? launch_wg<MINW, false, true>(pairs, wg, count, st) : launch_wg<MINW, false, false>(pairs, wg, count, st);
+}
+
+hipError_t launch_batch_icgn_wg(const PairDesc* pairs, const int2* wg, int count, bool table, int waves,
+                                bool xcd_ranges, hipStream_t st)
 {
     if (count <= 0) return hipSuccess;
-    if (table)
-        hipLaunchKernelGGL((k_batch_icgn_wg<TWDIC_INTERP_FUSED, 5, true>), dim3(count), dim3(256), 0, st, pairs, wg);
+    if (waves == 6)
+        launch_wg_w<6>(pairs, wg, count, table, xcd_ranges, st);
     else
-        hipLaunchKernelGGL((k_batch_icgn_wg<TWDIC_INTERP_FUSED, 5, false>), dim3(count), dim3(256), 0, st, pairs, wg);
+        launch_wg_w<5>(pairs, wg, count, table, xcd_ranges, st);
     return hipGetLastError();
 }
 

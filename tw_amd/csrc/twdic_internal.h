@@ -122,9 +122,11 @@ int mma_form(int N);
 hipError_t launch_batch_fftcc_mma(const PairDesc* pairs, const int2* items, int count, int form, hipStream_t st);
 // batched form: IC-GN (fused interpolation) over workgroup items (pair, first
 // POI), four POIs of one pair per workgroup; `table` when every listed pair's
-// subset fits the step table (step_table_fits)
+// subset fits the step table (step_table_fits).
 bool step_table_fits(int s);
-hipError_t launch_batch_icgn_wg(const PairDesc* pairs, const int2* wg, int count, bool table, hipStream_t st);
+// waves: occupancy target 5 or 6; xcd_ranges: contiguous per-XCD item ranges
+hipError_t launch_batch_icgn_wg(const PairDesc* pairs, const int2* wg, int count, bool table, int waves,
+                                bool xcd_ranges, hipStream_t st);
 hipError_t launch_ref_state(const IcgnParams& p, hipStream_t st);
 hipError_t launch_icgn(const IcgnParams& p, int interp, int variant, hipStream_t st);
 // persistent batch kernels: `blocks` workgroups drain a device work counter
