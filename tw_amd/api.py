@@ -217,6 +217,24 @@ def displacement(d: Deform, x: float, y: float) -> np.ndarray:
 
 
 # --------------------------------------------------------------------------- device
+# Bytes per POI of each per-POI buffer the kernels index: (y, x) int32 pairs,
+# float32 u / v / ZNCC / |dp| / accumulators, 6 float32 warp parameters, int32
+# iteration counts, uint8 flags and FFT-CC status.
+_PER_POI = {"poi": 8, "out": 8, "u": 4, "v": 4, "zncc": 4, "dpnorm": 4, "acc_u": 4, "acc_v": 4, "p6": 24,
+            "iters": 4, "flags": 1, "status": 1}
+
+
+def _fits(n: int, **bufs) -> None:
+    """Host-side bounds check before a launch.  The C ABI takes plain pointers
+    and cannot see sizes, so every per-POI buffer given here must hold n
+    entries, or the call raises before anything reaches the GPU."""
+    if n < 0:
+        raise ValueError(f"negative POI count {n}")
+    for name, a in bufs.items():
+        if a is not None and a.nbytes < n * _PER_POI[name]:
+            raise ValueError(f"{name}: {a.nbytes} bytes hold fewer than {n} POIs")
+
+
 class DeviceArray:
     """A device allocation made through twdic_malloc, freed with the context."""
 
@@ -315,22 +333,26 @@ class Context:
 
     def fftcc(self, poi: DeviceArray, n: int, u: DeviceArray, v: DeviceArray, zncc: DeviceArray,
               status: DeviceArray | None = None):
+        _fits(n, poi=poi, u=u, v=v, zncc=zncc, status=status)
         self._check(lib().twdic_fftcc(self.h, poi.ptr, n, u.ptr, v.ptr, zncc.ptr,
                                       status.ptr if status is not None else None), "fftcc")
 
     def icgn(self, poi: DeviceArray, n: int, u: DeviceArray, v: DeviceArray, p6=None, iters=None,
              flags=None, dpnorm=None):
+        _fits(n, poi=poi, u=u, v=v, p6=p6, iters=iters, flags=flags, dpnorm=dpnorm)
         g = lambda a: a.ptr if a is not None else None  # noqa: E731
         self._check(lib().twdic_icgn(self.h, poi.ptr, n, u.ptr, v.ptr, g(p6), g(iters), g(flags), g(dpnorm)),
                     "icgn")
 
     def frame(self, poi, n, u, v, zncc, p6=None, iters=None, flags=None):
+        _fits(n, poi=poi, u=u, v=v, zncc=zncc, p6=p6, iters=iters, flags=flags)
         g = lambda a: a.ptr if a is not None else None  # noqa: E731
         self._check(lib().twdic_frame(self.h, poi.ptr, n, u.ptr, v.ptr, zncc.ptr, g(p6), g(iters), g(flags)),
                     "frame")
 
     def frame_capture(self, poi, n, u, v, zncc, p6=None, iters=None, flags=None):
         """Record the frame step over these buffers into the context's HIP graph."""
+        _fits(n, poi=poi, u=u, v=v, zncc=zncc, p6=p6, iters=iters, flags=flags)
         g = lambda a: a.ptr if a is not None else None  # noqa: E731
         self._check(lib().twdic_frame_capture(self.h, poi.ptr, n, u.ptr, v.ptr, zncc.ptr, g(p6), g(iters),
                                               g(flags)), "frame_capture")
@@ -355,12 +377,15 @@ class Context:
         return out
 
     def track_update_poi(self, poi: DeviceArray, n: int, u: DeviceArray, v: DeviceArray):
+        _fits(n, poi=poi, u=u, v=v)
         self._check(lib().twdic_track_update_poi(self.h, poi.ptr, n, u.ptr, v.ptr), "track_update_poi")
 
     def track_current_poi(self, poi: DeviceArray, n: int, u: DeviceArray, v: DeviceArray, out: DeviceArray):
+        _fits(n, poi=poi, u=u, v=v, out=out)
         self._check(lib().twdic_track_current_poi(self.h, poi.ptr, n, u.ptr, v.ptr, out.ptr), "track_current_poi")
 
     def track_accumulate_uv(self, n: int, u: DeviceArray, v: DeviceArray, acc_u: DeviceArray, acc_v: DeviceArray):
+        _fits(n, u=u, v=v, acc_u=acc_u, acc_v=acc_v)
         self._check(lib().twdic_track_accumulate_uv(self.h, n, u.ptr, v.ptr, acc_u.ptr, acc_v.ptr),
                     "track_accumulate_uv")
 
