@@ -105,3 +105,69 @@ def test_frame_loop_against_oracle(tmp_path):
     assert np.all(res[2]["u"] == 0) and np.all(res[2]["v"] == 0) and np.all(res[2]["iters"] == 1)
     # the POIs did move (frame 2's u is about -3 px)
     assert np.mean(moved[:, 1] - poi0[:, 1]) < -1.5
+
+
+WORKER = api.LIB_PATH.parent / "tw_icgn_worker"
+
+
+def test_icgn_worker_host_arrays_against_oracle(tmp_path):
+    """TW_Core's IC-GN worker (ICGNWorkerThread::processFrame,
+    icgnworkerthread.cpp:45-73) with cuICGN2D in place of ICGN2D_CPU: per
+    frame ResetRefImg, then ICGN2D_Algorithm on host arrays of POIs and
+    integer guesses.  Two frames, the second on a new reference.  Bars as in
+    the parity tests: flags equal; u, v within 1e-4 px of the FP32 oracle and
+    5e-5 px of the FP64 oracle where the iteration counts agree; counts differ
+    only where the oracle stopped with |dp| within 5 % of delta_p, at most
+    max(2, 0.2 %) POIs or three times the FP32 oracle's own flips against
+    FP64, + 2."""
+    W = H = 320
+    d1 = api.affine(2.4, -1.6, 0.004, -0.003, 0.001, 159.5, 159.5)
+    d2 = api.affine(-1.2, 3.3, -0.002, 0.002, 0.0, 159.5, 159.5)
+    ref, t1 = api.synth_pair(W, H, 1500, 41, d1)
+    _, t2 = api.synth_pair(W, H, 1500, 41, d2, want_ref=False)
+    roi, S, G, M, N = (1, 1, W - 2, H - 2), 15, 7, 10, 32
+    case = cases.Case("worker", W, S, G, N, roi, M, ref, None, None)
+    poi = case.poi()
+    n = len(poi)
+    nx, ny = len(np.unique(poi[:, 1])), len(np.unique(poi[:, 0]))
+    assert nx * ny == n
+    frames = [(ref, t1), (t1, t2)]
+    guesses = []
+    for R, T in frames:
+        ex = oracle.fftcc_exact(R, T, poi, N)
+        guesses.append((ex["u"].astype(np.float32), ex["v"].astype(np.float32)))
+    (tmp_path / "poi.bin").write_bytes(np.ascontiguousarray(poi, np.int32).tobytes())
+    (tmp_path / "uv0.bin").write_bytes(b"".join(u.tobytes() + v.tobytes() for u, v in guesses))
+    paths = []
+    for k, (R, T) in enumerate(frames):
+        for tag, img in (("r", R), ("t", T)):
+            p = tmp_path / f"{tag}{k}.raw"
+            img.tofile(p)
+            paths.append(str(p))
+    out = tmp_path / "out.bin"
+    r = subprocess.run([str(WORKER), str(W), str(H), *map(str, roi), str(S), str(nx), str(ny),
+                        str(tmp_path / "poi.bin"), str(tmp_path / "uv0.bin"), str(out), *paths],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    raw = out.read_bytes()
+    per = 13 * n
+    assert len(raw) == per * len(frames)
+    for k, ((R, T), (u0, v0)) in enumerate(zip(frames, guesses)):
+        b = raw[k * per:(k + 1) * per]
+        gu = np.frombuffer(b[:4 * n], np.float32)
+        gv = np.frombuffer(b[4 * n:8 * n], np.float32)
+        gi = np.frombuffer(b[8 * n:12 * n], np.int32)
+        gf = np.frombuffer(b[12 * n:], np.uint8)
+        o = oracle.icgn(R, T, roi, S, S, poi, u0, v0)
+        o64 = oracle.icgn(R, T, roi, S, S, poi, u0.astype(np.float64), v0.astype(np.float64), precision="f64")
+        assert np.array_equal(gf, o["flags"]), k
+        assert np.mean(gf == 0) > 0.95, k
+        longer = gi > o["iters"]
+        assert np.all(np.abs(o["dpnorm"][longer] / 1e-3 - 1.0) < 0.05), k
+        own = int(np.count_nonzero(o["iters"] != o64["iters"]))     # the FP32 path's own flips
+        assert np.count_nonzero(gi != o["iters"]) <= max(2, n // 500, 3 * own + 2), k
+        ok = (gi == o["iters"]) & (o["flags"] == 0)
+        assert np.max(np.abs(gu[ok] - o["u"][ok])) <= 1e-4 and np.max(np.abs(gv[ok] - o["v"][ok])) <= 1e-4, k
+        ok64 = ok & (gi == o64["iters"])
+        assert np.max(np.abs(gu[ok64] - o64["u"][ok64])) <= 5e-5, k
+        assert np.max(np.abs(gv[ok64] - o64["v"][ok64])) <= 5e-5, k

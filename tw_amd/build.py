@@ -72,12 +72,13 @@ def build(verbose: bool = False) -> Path:
     tmp = LIB.with_suffix(".so.tmp")
     _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", str(tmp), *objs, "-lgomp"])
     os.replace(tmp, LIB)
-    # the TW_Core-style frame driver, linked against the library in-tree
-    exe = OUT_DIR / "tw_frame_loop"
-    _run([HIPCC, "-O2", "-std=c++17", f"-I{INCLUDE}", str(CPPSRC / "examples" / "frame_loop.cpp"),
-          "-o", str(exe), f"-L{OUT_DIR}", "-ltwdic", "-Wl,-rpath,$ORIGIN"])
+    # and TW_Core-style worker drivers, linked against the library in-tree
+    exes = [(OUT_DIR / "tw_frame_loop", "frame_loop.cpp"), (OUT_DIR / "tw_icgn_worker", "icgn_worker.cpp")]
+    for exe, src in exes:
+        _run([HIPCC, "-O2", "-std=c++17", f"-I{INCLUDE}", str(CPPSRC / "examples" / src),
+              "-o", str(exe), f"-L{OUT_DIR}", "-ltwdic", "-Wl,-rpath,$ORIGIN"])
     if verbose:
-        print(f"built {LIB} and {exe}")
+        print(f"built {LIB} and {', '.join(str(e) for e, _ in exes)}")
     return LIB
 
 
