@@ -315,10 +315,14 @@ class Context:
         self._check(lib().twdic_set_reference(self.h, _ptr(img), img.shape[1], 0), "set_reference")
         self.sync()
 
-    def set_target(self, img: np.ndarray):
+    def set_target(self, img: np.ndarray, sync: bool = True):
+        """Host image in: copied into pinned staging before the call returns
+        (img may be reused at once), then one DMA on the context's stream.
+        sync=False leaves that DMA in flight behind the work already queued."""
         img = np.ascontiguousarray(img, np.uint8)
         self._check(lib().twdic_set_target(self.h, _ptr(img), img.shape[1], 0), "set_target")
-        self.sync()
+        if sync:
+            self.sync()
 
     def set_reference_device(self, d: DeviceArray, stride: int):
         self._check(lib().twdic_set_reference(self.h, d.ptr, stride, 1), "set_reference")
