@@ -128,3 +128,23 @@ def test_synth_speckle_statistics(lib_built):
     c = np.real(np.fft.ifft2(np.conj(F) * G))
     k = np.unravel_index(np.argmax(c), c.shape)
     assert (k[1], k[0] - 512) == (2, -2)
+
+
+def test_poi_count_bounds_check_on_the_host():
+    """tw_amd.api refuses a POI count its per-POI buffers cannot hold, before
+    any call into the library (the C ABI sees plain pointers only)."""
+    from types import SimpleNamespace as B
+
+    from tw_amd import api
+    n = 10
+    ok = dict(poi=B(nbytes=8 * n), u=B(nbytes=4 * n), v=B(nbytes=4 * n), p6=B(nbytes=24 * n),
+              iters=B(nbytes=4 * n), flags=B(nbytes=n))
+    api._fits(n, **ok)
+    api._fits(0, **ok)
+    api._fits(n, poi=ok["poi"], u=ok["u"], v=ok["v"], p6=None, flags=None)   # optional outputs absent
+    for k in ok:
+        short = dict(ok, **{k: B(nbytes=ok[k].nbytes - 1)})
+        with pytest.raises(ValueError, match=k):
+            api._fits(n, **short)
+    with pytest.raises(ValueError):
+        api._fits(-1, **ok)
