@@ -64,91 +64,99 @@ def copy_calibration(mib: int, reps: int):
     print(json.dumps({"copy_bytes": nbytes, "reps": reps}))
 
 
-def serving(a):
-    """Poisson stream of correlation requests through the serving loop: 1024^2
-    frames deformed from a resident reference (strains within +-0.5 %,
-    translations within +-4 px), POI sets of 500..20,000 POIs with subset
-    radius 10, 15 or 20 (N = 32, 32, 64), the reference replaced every 200
-    requests.  Frames and POI sets come from pre-generated pools (8 targets
-    per reference, 4 references, 16 POI sets)."""
+def run_serving(rate, duration, slots, frame=1024, classes=((10, 32), (15, 32), (20, 64)), poisets=None,
+                queue_capacity=1024, seed=1005, refs_n=4, tars_per_ref=8, ref_every=200, closed_loop=0):
+    """Correlation requests through the serving loop: frame x frame targets
+    deformed from a resident reference (strains within +-0.5 %, translations
+    within +-4 px), POI sets from a pool, subset radius and FFT-CC window per
+    request from `classes`, the reference replaced every `ref_every`
+    requests.  Frames and POI sets come from pre-generated pools.  Arrivals
+    are Poisson at `rate` per second for `duration` seconds (dropped when the
+    admission queue is full), or, with closed_loop = K, K requests submitted
+    back to back as fast as the queue admits them (blocking)."""
     from tw_amd import api
-    W = H = 1024
-    rng = np.random.default_rng(1005)
+    W = H = frame
+    rng = np.random.default_rng(seed)
     t_gen = time.perf_counter()
     refs, tars = [], []
-    for r in range(4):
-        seed = 1005900000 + r
+    for r in range(refs_n):
         ref = None
         pool = []
-        for k in range(8):
+        for k in range(tars_per_ref):
             d = api.affine(*rng.uniform(-4, 4, 2), *rng.uniform(-0.005, 0.005, 3), W / 2, H / 2)
-            rr, t = api.synth_pair(W, H, int(0.0153 * W * H), seed, d, noise_sigma=2.0,
-                                   noise_seed=1005000000 + 8 * r + k, want_ref=ref is None)
+            rr, t = api.synth_pair(W, H, int(0.0153 * W * H), 1005900000 + r, d, noise_sigma=2.0,
+                                   noise_seed=1005000000 + tars_per_ref * r + k, want_ref=ref is None)
             ref = rr if ref is None else ref
             pool.append(t)
         refs.append(ref)
         tars.append(pool)
-    sets = []
-    for k in range(16):
-        n = int(rng.integers(500, 20001))
-        sets.append(np.stack([rng.integers(40, H - 40, n), rng.integers(40, W - 40, n)], 1).astype(np.int32))
+    if poisets is None:
+        poisets = []
+        for k in range(16):
+            n = int(rng.integers(500, 20001))
+            poisets.append(np.stack([rng.integers(40, H - 40, n), rng.integers(40, W - 40, n)], 1).astype(np.int32))
     t_gen = time.perf_counter() - t_gen
-    classes = [(10, 32), (15, 32), (20, 64)]
-    with api.Server(slots=a.slots, queue_capacity=1024, max_width=W, max_height=H, max_pois=20000) as srv:
-        ids = [srv.add_poiset(p) for p in sets]
-        outs = [[api.Server.outputs(len(p)) for _ in range(4)] for p in sets]
+    max_pois = max(len(p) for p in poisets)
+    with api.Server(slots=slots, queue_capacity=queue_capacity, max_width=W, max_height=H,
+                    max_pois=max_pois) as srv:
+        ids = [srv.add_poiset(p) for p in poisets]
+        outs = [[api.Server.outputs(len(p)) for _ in range(4)] for p in poisets]
         cur = 0
         srv.set_reference(0, refs[cur])
         # warm-up: one request of every class and set
-        for i in range(len(sets)):
+        for i in range(len(poisets)):
             for S, N in classes:
                 srv.submit(0, ids[i], tars[cur][0], S, N, out=outs[i][0])
         srv.drain()
         srv.stats_reset()
-        n_req = int(a.rate * a.duration)
-        gaps = rng.exponential(1.0 / a.rate, n_req)
+        n_req = closed_loop if closed_loop else int(rate * duration)
+        gaps = rng.exponential(1.0 / rate, n_req) if not closed_loop else np.zeros(n_req)
         t_start = time.perf_counter()
         t_next = t_start
-        submitted = 0
         for k in range(n_req):
             t_next += gaps[k]
-            while True:
+            while not closed_loop:
                 now = time.perf_counter()
                 if now >= t_next:
                     break
                 if t_next - now > 2e-4:
                     time.sleep(t_next - now - 1e-4)
-            if k and k % 200 == 0:                     # new reference every 200 requests
+            if k and k % ref_every == 0:
                 cur = (cur + 1) % len(refs)
                 srv.set_reference(0, refs[cur])
-            i = int(rng.integers(0, len(sets)))
-            S, N = classes[int(rng.integers(0, 3))]
-            t, _ = srv.submit(0, ids[i], tars[cur][int(rng.integers(0, 8))], S, N, drop_if_full=True,
-                              out=outs[i][k % 4])
-            submitted += t is not None
+            i = int(rng.integers(0, len(poisets)))
+            S, N = classes[int(rng.integers(0, len(classes)))]
+            srv.submit(0, ids[i], tars[cur][int(rng.integers(0, tars_per_ref))], S, N,
+                       drop_if_full=not closed_loop, out=outs[i][k % 4])
         t_arrivals = time.perf_counter() - t_start
         srv.drain()
         t_all = time.perf_counter() - t_start
         st = srv.stats()
+    return {"value": st.pois / t_all, "unit": "POIs/s", "offered_requests": n_req, "arrival_s": t_arrivals,
+            "wall_s": t_all, "completed": int(st.completed), "dropped": int(st.dropped), "failed": int(st.failed),
+            "requests_per_s": st.completed / t_all,
+            "latency_ms": {"p50": st.lat_p50_ms, "p99": st.lat_p99_ms, "max": st.lat_max_ms},
+            "slots_busy_peak": st.slots_busy_peak, "hbm_peak_bytes": int(st.hbm_peak),
+            "state_builds": int(st.state_builds), "state_hits": int(st.state_hits), "synth_s": t_gen}
+
+
+def serving(a):
+    """--workload serving: the serving configuration's Poisson stream (1024^2
+    frames, POI sets of 500..20,000 POIs, S in {10, 15, 20}, 2000 req/s)."""
+    r = run_serving(a.rate, a.duration, a.slots)
     out = {
         "metric": "serving: IC-GN POIs/s and p50/p99 request latency under a Poisson request stream",
-        "value": st.pois / t_all, "unit": "POIs/s", "n_gpus": 1, "higher_is_better": True,
-        "dtype": "fp32", "data": "synthetic",
+        "n_gpus": 1, "higher_is_better": True, "dtype": "fp32", "data": "synthetic",
         "config": {"workload": "serving_1024_poisson", "rate_req_s": a.rate, "duration_s": a.duration,
                    "slots": a.slots, "frame": "1024x1024 uint8 (host memory)",
                    "pois_per_request": "500..20000 (16-set pool)",
                    "subset_radius": [10, 15, 20], "reference_every": 200, "queue_capacity": 1024},
-        "offered_requests": n_req, "arrival_s": t_arrivals, "wall_s": t_all,
-        "completed": int(st.completed), "dropped": int(st.dropped), "failed": int(st.failed),
-        "requests_per_s": st.completed / t_all,
-        "latency_ms": {"p50": st.lat_p50_ms, "p99": st.lat_p99_ms, "max": st.lat_max_ms},
-        "slots_busy_peak": st.slots_busy_peak, "hbm_peak_bytes": int(st.hbm_peak),
-        "state_builds": int(st.state_builds), "state_hits": int(st.state_hits), "synth_s": t_gen,
     }
+    out.update(r)
     print(json.dumps(out))
 
 
-def batch(a):
+def run_batch(steps, warmup, tune=True, pairs_n=64, progress=True):
     """The batched configuration: 64 speckle pairs of sizes drawn from
     {512, 1024, 2048, 4096}^2 (0.015 spots/px^2), each with a random affine
     field (strains within +-1 %, translations within +-8 px), subset radius
@@ -157,58 +165,54 @@ def batch(a):
     ceil(8 + size/100)) px (the largest displacement the field reaches).  One
     step is FFT-CC then IC-GN over every POI of every pair, through the
     batched API (flat form: one launch per stage and kernel form over all
-    pairs; TWDIC_BATCH_FORM=direct runs the single-context kernels per pair)."""
+    pairs; TWDIC_BATCH_FORM=direct runs the single-context kernels per pair).
+    The plain per-frame form of the same work runs every pair alone, one
+    after another, through the single-context kernels."""
+    from tests import cases
     from tw_amd import api
-    rng = np.random.default_rng(1003)
     t_gen = time.perf_counter()
+    say = (lambda i, p: (i % 16 == 15) and print(f"batch: {i + 1} pairs synthesised", file=sys.stderr,
+                                                 flush=True)) if progress else None
+    ps = cases.batch_pairs(count=pairs_n, on_pair=say)
     alloc = api.Context(api.make_config(64, 64, 1, 1, 62, 62, 2, 2, 4, 4, 0, 0, 8))
     pairs, keep, n_tot, classes = [], [], 0, {}
-    for i in range(a.pairs):
-        size = int(rng.choice([512, 1024, 2048, 4096]))
-        S = int(rng.choice([10, 15, 20, 30]))
-        g = int(rng.choice([4, 5, 10]))
-        N = 32 if S <= 15 else 64
-        d = api.affine(*rng.uniform(-8, 8, 2), *rng.uniform(-0.01, 0.01, 3), size / 2, size / 2)
-        ref, tar = api.synth_pair(size, size, int(0.015 * size * size), 1003000 + i, d, noise_sigma=2.0)
-        m = max(N // 2, S + 2 + int(np.ceil(8 + 0.01 * size)))
-        ys = np.arange(m, size - m, g)
-        yy, xx = np.meshgrid(ys, ys, indexing="ij")
-        poi = np.stack([yy.ravel(), xx.ravel()], 1).astype(np.int32)
-        n = len(poi)
-        b = dict(ref=alloc.array(ref.shape, np.uint8).upload(ref), tar=alloc.array(tar.shape, np.uint8).upload(tar),
-                 poi=alloc.array(poi.shape, np.int32).upload(poi), u=alloc.array(n, np.float32),
+    for p in ps:
+        size, n = p.size, len(p.poi)
+        b = dict(ref=alloc.array(p.ref.shape, np.uint8).upload(p.ref),
+                 tar=alloc.array(p.tar.shape, np.uint8).upload(p.tar),
+                 poi=alloc.array(p.poi.shape, np.int32).upload(p.poi), u=alloc.array(n, np.float32),
                  v=alloc.array(n, np.float32), p6=alloc.array((n, 6), np.float32), it=alloc.array(n, np.int32),
                  fl=alloc.array(n, np.uint8))
         keep.append(b)
-        pairs.append(api.Pair(b["ref"].ptr, b["tar"].ptr, size, size, size, 1, 1, size - 2, size - 2, S, S, N, 20,
-                              1e-3, b["poi"].ptr, n, b["u"].ptr, b["v"].ptr, None, b["p6"].ptr, b["it"].ptr,
+        pairs.append(api.Pair(b["ref"].ptr, b["tar"].ptr, size, size, size, 1, 1, size - 2, size - 2, p.S, p.S, p.N,
+                              20, 1e-3, b["poi"].ptr, n, b["u"].ptr, b["v"].ptr, None, b["p6"].ptr, b["it"].ptr,
                               b["fl"].ptr))
         n_tot += n
-        classes[f"{size}^2 S{S} g{g}"] = classes.get(f"{size}^2 S{S} g{g}", 0) + 1
-        if i % 8 == 7:
-            print(f"batch: {i + 1} pairs synthesised, {n_tot} POIs", file=sys.stderr, flush=True)
+        key = f"{size}^2 S{p.S} g{p.grid}"
+        classes[key] = classes.get(key, 0) + 1
     alloc.sync()
     t_gen = time.perf_counter() - t_gen
     tuned = None
+    flat = os.environ.get("TWDIC_BATCH_FORM") != "direct"
     with api.Batch() as bt:
         bt.set_pairs(pairs)
         bt.run()                                   # reference state + first frame
         bt.sync()
-        if os.environ.get("TWDIC_BATCH_FORM") != "direct":
+        if flat and tune:
             bt.autotune(None, reps=2)              # flat form: IC-GN occupancy and XCD order
             t = bt.tuning
             tuned = {"icgn_waves": t.icgn_waves, "icgn_xcd_ranges": t.icgn_xcd_ranges, "use_graph": t.use_graph}
-        for _ in range(a.warmup_batch):
+        for _ in range(warmup):
             bt.run()
         bt.sync()
         bt.record(0)
-        for _ in range(a.steps_batch):
+        for _ in range(steps):
             bt.run()
         bt.record(1)
         bt.sync()
-        ms = bt.elapsed_ms(0, 1) / a.steps_batch
+        ms = bt.elapsed_ms(0, 1) / steps
         lat = []
-        for _ in range(a.steps_batch):
+        for _ in range(steps):
             bt.record(2)
             bt.run()
             bt.record(3)
@@ -242,24 +246,28 @@ def batch(a):
         else:
             os.environ["TWDIC_BATCH_FORM"] = form
     fl = np.concatenate([b["fl"].download() for b in keep])
-    out = {
-        "metric": "batched form: POIs/s over 64 mixed pairs (FFT-CC + IC-GN)",
-        "value": n_tot / (ms * 1e-3), "unit": "POIs/s", "n_gpus": 1, "steps": a.steps_batch,
-        "warmup": a.warmup_batch, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": "fp32", "data": "synthetic",
-        "config": {"workload": "batched64_mixed", "pairs": a.pairs, "pois": n_tot, "classes": classes,
-                   "execution": ("direct form (per-pair single-context kernels on four streams)"
-                                 if os.environ.get("TWDIC_BATCH_FORM") == "direct" else
-                                 "flat form (one launch per stage and kernel form over all pairs)")},
-        "icgn_pois_per_s": n_tot / (icgn_ms * 1e-3),
-        "latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99))},
-        "tuning": tuned,
-        "plain_form": {"ms_per_step": plain_ms, "pois_per_s": n_tot / (plain_ms * 1e-3),
-                       "batched_over_plain": plain_ms / ms,
-                       "note": "the same pairs run one at a time through the single-context kernels"},
-        "flags": np.bincount(fl, minlength=5).tolist(), "synth_s": t_gen,
-    }
     alloc.close()
+    return {"value": n_tot / (ms * 1e-3), "unit": "POIs/s", "steps": steps, "warmup": warmup, "ms_per_step": ms,
+            "pairs": pairs_n, "pois": n_tot, "classes": classes,
+            "execution": ("flat form (one launch per stage and kernel form over all pairs)" if flat else
+                          "direct form (per-pair single-context kernels on four streams)"),
+            "icgn_pois_per_s": n_tot / (icgn_ms * 1e-3),
+            "latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99))},
+            "tuning": tuned,
+            "plain_form": {"ms_per_step": plain_ms, "pois_per_s": n_tot / (plain_ms * 1e-3),
+                           "note": "the same pairs run one at a time through the single-context kernels"},
+            "batched_over_plain": plain_ms / ms,
+            "flags": np.bincount(fl, minlength=5).tolist(), "synth_s": t_gen}
+
+
+def batch(a):
+    """--workload batch: the batched configuration, full line."""
+    r = run_batch(a.steps_batch, a.warmup_batch)
+    out = {"metric": "batched form: POIs/s over 64 mixed pairs (FFT-CC + IC-GN)", "n_gpus": 1,
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "fp32", "data": "synthetic",
+           "config": {"workload": "batched64_mixed", "pairs": r["pairs"], "pois": r["pois"], "classes": r["classes"],
+                      "execution": r["execution"]}}
+    out.update({k: v for k, v in r.items() if k not in ("pairs", "pois", "classes", "execution")})
     print(json.dumps(out))
 
 
@@ -273,8 +281,10 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-reference-window", action="store_true",
                     help="skip the secondary frame timing at the reference's FFT-CC window N = 2S")
-    ap.add_argument("--cpu-sample", type=int, default=0,
-                    help="POIs in the CPU baseline sample (0: the whole frame, about 15 s on 16 cores)")
+    ap.add_argument("--cpu-sample", type=int, default=16384,
+                    help="POIs in the CPU baseline sample (0: the whole frame, about 15 s per run on 16 cores)")
+    ap.add_argument("--no-batched", action="store_true", help="skip the compact batched-configuration leg")
+    ap.add_argument("--no-served", action="store_true", help="skip the compact serving legs")
     ap.add_argument("--workload", choices=["frame", "serving", "batch"], default="frame",
                     help="frame: the 2048^2 plain frame step (default); serving: Poisson request stream "
                          "through the serving loop; batch: 64 mixed pairs through the batched API")
@@ -289,32 +299,63 @@ def parse():
     return ap.parse_args()
 
 
-def cpu_baseline(case, poi, sample):
+def cpu_model() -> str:
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(case, poi, sample, runs=5):
     """The CPU oracle (FP32 restatement of the reference CPU path, OpenMP over
-    POIs on all host cores) on a bounded sample of the same workload:
-    whole-ROI precompute + FFT-CC and IC-GN on `sample` evenly spaced POIs,
-    scaled to POIs/s of a full frame."""
+    POIs and over precompute rows on all host cores, as the reference's
+    Multicore variant) on a bounded sample of the same workload, timed as the
+    reference's harness does (TW_EngineTester/FFTCC+ICGN.cpp:45,84-97): one
+    warm-up run, then the mean of `runs` runs, with FFT-CC, the whole-ROI
+    precompute (gradients + LUT) and the IC-GN loop timed separately.  The
+    frame rate is scaled from the sample to every POI of the frame (the
+    precompute is per frame and is counted once).  Also the 512^2 workload
+    (2,304 POIs, 31 x 31 subsets, N = 32), whole, by the same procedure."""
     import oracle
-    if sample <= 0 or sample >= len(poi):
-        sample = len(poi)
-    idx = np.linspace(0, len(poi) - 1, sample).astype(int)
-    sp = np.ascontiguousarray(poi[idx])
+    from tests import cases
     cores = len(os.sched_getaffinity(0))
     if os.environ.get("OMP_NUM_THREADS", "").isdigit():
         cores = min(cores, int(os.environ["OMP_NUM_THREADS"]))
-    t0 = time.perf_counter()
-    f = oracle.fftcc(case.ref, case.tar, sp, case.N, nthreads=cores)
-    t1 = time.perf_counter()
-    o = oracle.icgn(case.ref, case.tar, case.roi, case.S, case.S, sp, f["u"], f["v"], nthreads=cores)
-    t_pre = o["times_ms"][0] / 1e3
-    t_loop = o["times_ms"][1] / 1e3
-    t_fft = t1 - t0
-    n = len(poi)
-    frame_s = t_pre + (t_fft + t_loop) * n / sample
-    return {"value": n / frame_s, "unit": "POIs/s", "cores": cores, "kind": "port",
-            "sample": f"{sample} of {n} POIs (FFT-CC + IC-GN) plus whole-ROI gradients/LUT; "
-                      f"fftcc {t_fft:.2f}s, icgn {t_loop:.2f}s, precompute {t_pre:.2f}s",
-            "fftcc_pois_per_s": sample / t_fft, "icgn_pois_per_s": sample / t_loop}
+
+    def timed(c, pts):
+        t0 = time.perf_counter()
+        f = oracle.fftcc(c.ref, c.tar, pts, c.N, nthreads=cores)
+        t1 = time.perf_counter()
+        o = oracle.icgn(c.ref, c.tar, c.roi, c.S, c.S, pts, f["u"], f["v"], nthreads=cores)
+        return t1 - t0, o["times_ms"][0] / 1e3, o["times_ms"][1] / 1e3
+
+    def measure(c, pts, n_frame):
+        timed(c, pts)                                    # warm-up
+        t = np.array([timed(c, pts) for _ in range(runs)])
+        t_fft, t_pre, t_loop = t.mean(0)
+        frame_s = t_pre + (t_fft + t_loop) * n_frame / len(pts)
+        return {"value": n_frame / frame_s, "frames_per_s": 1.0 / frame_s,
+                "fftcc_pois_per_s": len(pts) / t_fft, "icgn_pois_per_s": len(pts) / t_loop,
+                "icgn_with_precompute_pois_per_s": n_frame / (t_pre + t_loop * n_frame / len(pts)),
+                "seconds": {"fftcc": t_fft, "precompute": t_pre, "icgn": t_loop}}
+
+    if sample <= 0 or sample >= len(poi):
+        sample = len(poi)
+    idx = np.linspace(0, len(poi) - 1, sample).astype(int)
+    main = measure(case, np.ascontiguousarray(poi[idx]), len(poi))
+    c1 = cases.speckle_512()
+    c1_line = measure(c1, c1.poi(), len(c1.poi()))
+    c1_line.update({"unit": "POIs/s", "workload": "speckle512_S15_grid10_N32 (2,304 POIs, whole)"})
+    out = {"value": main["value"], "unit": "POIs/s", "cores": cores, "kind": "port", "cpu_model": cpu_model(),
+           "sample": f"{sample} of {len(poi)} POIs evenly spaced (FFT-CC + IC-GN) plus the whole-ROI "
+                     f"gradients/LUT per frame; 1 warm-up, mean of {runs} runs; own mixed-radix FFT and "
+                     f"textbook Cholesky (MKL/FFTW absent)"}
+    out.update({k: v for k, v in main.items() if k != "value"})
+    out["speckle512"] = c1_line
+    return out
 
 
 def main():
@@ -480,7 +521,7 @@ def main():
                                        "algorithmic_flops_per_launch": d["flops"]},
                     "hbm": {"achieved": d["gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
                             "frac": d["gbs"] / HBM_PEAK_GBS, "algorithmic_bytes_per_launch": d["bytes"]}}
-        return {"bound": "mfma", "pipe": "fp32/fp64 VALU (IC-GN has no dense contraction for the matrix cores)",
+        return {"bound": "valu", "pipe": "fp32/fp64 VALU (IC-GN has no dense contraction for the matrix cores)",
                 "kernel": name, "achieved": d["tflops"], "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
                 "frac": d["tflops"] / FP32_PEAK_TFLOPS,
                 "peak_without_packed_fp32": FP32_UNPACKED_PEAK_TFLOPS,
@@ -500,7 +541,9 @@ def main():
     fps = 1e3 / ms_step
     frame_roof = {
         "hbm": {"algorithmic_bytes_per_frame": b_frame, "achieved": b_frame * fps / 1e9, "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": b_frame * fps / 1e9 / HBM_PEAK_GBS},
+                "unit": "GB/s", "frac": b_frame * fps / 1e9 / HBM_PEAK_GBS,
+                "note": "a convention figure, not traffic: it counts the 16-coefficient LUT and the Rx/Ry planes "
+                        "the reference materialises, which this design never writes"},
         "valu": {"algorithmic_flops_per_poi": f_poi, "achieved": f_poi * n * fps / 1e12, "peak": FP32_PEAK_TFLOPS,
                  "unit": "TFLOP/s", "frac": f_poi * n * fps / 1e12 / FP32_PEAK_TFLOPS},
     }
@@ -546,9 +589,44 @@ def main():
     }
     if not a.no_reference_window:
         out["reference_window"] = reference_window(case, interp, dref, dtar, W, H)
-    if not a.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(case, pyx, a.cpu_sample)
     ctx.close()
+    plain = {"pois_per_s": out["value"], "ms_per_step": ms_step,
+             "host_buffers_pois_per_s": out["host_buffers"]["pois_per_s"]}
+    if not a.no_batched:
+        print("bench: batched configuration", file=sys.stderr, flush=True)
+        r = run_batch(3, 1, tune=False)
+        out["batched"] = {k: r[k] for k in ("value", "unit", "ms_per_step", "pois", "icgn_pois_per_s", "latency_ms",
+                                            "plain_form", "batched_over_plain", "execution", "steps")}
+        out["batched"]["workload"] = "batched64_mixed (64 pairs of 512^2..4096^2, S 10..30, 10.55M POIs)"
+    if not a.no_served:
+        print("bench: serving, Poisson 2000 req/s", file=sys.stderr, flush=True)
+        r = run_serving(2000.0, 5.0, 8)
+        out["served"] = {"poisson_1024": dict(r, workload="1024^2 frames, 500..20,000 POIs, S 10/15/20, "
+                                                           "2000 req/s Poisson for 5 s, 8 slots")}
+        print("bench: serving, 2048^2 frames", file=sys.stderr, flush=True)
+        r2 = run_serving(1.0, 1.0, 4, frame=2048, classes=((case.S, case.N),), poisets=[pyx], queue_capacity=8,
+                         refs_n=1, tars_per_ref=4, ref_every=10 ** 9, closed_loop=120)
+        r2["workload"] = ("2048^2 frames with the headline POI grid (157,609 POIs, 41x41, N 64), 120 requests "
+                          "submitted back to back (closed loop, queue 8, 4 slots), host-memory frames and outputs")
+        r2["served_over_plain"] = r2["value"] / plain["pois_per_s"]
+        r2["served_over_plain_host_buffers"] = r2["value"] / plain["host_buffers_pois_per_s"]
+        out["served"]["c2_frames"] = r2
+    # the 3x batched/served-over-plain target at 2048^2 with 41x41 subsets
+    tgt = {"target": "batched/served >= 3x the plain per-frame form's POIs/s at 2048^2, 41x41",
+           "plain_pois_per_s": plain["pois_per_s"]}
+    if "batched" in out:
+        tgt["batched_over_plain_mixed_pairs"] = out["batched"]["batched_over_plain"]
+    if "served" in out:
+        tgt["served_over_plain_2048"] = out["served"]["c2_frames"]["served_over_plain"]
+    tgt["met"] = False
+    tgt["why"] = ("one 2048^2 frame (157,609 POIs, 39k IC-GN workgroups) already fills the GPU, and the plain form "
+                  "already keeps the per-reference state (gradients, Hessian factors) resident across frames; "
+                  "batching and serving remove launch gaps, host transfers and small-frame underfill, but the "
+                  "per-POI kernel cost bounds every form")
+    out["target_3x"] = tgt
+    if not a.no_cpu_baseline:
+        print("bench: CPU baseline", file=sys.stderr, flush=True)
+        out["cpu_baseline"] = cpu_baseline(case, pyx, a.cpu_sample)
     print(json.dumps(out))
 
 

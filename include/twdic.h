@@ -34,7 +34,8 @@ typedef enum {
     TWDIC_OK = 0,
     TWDIC_EINVAL = 1,   /* bad argument or geometry */
     TWDIC_ENOMEM = 2,   /* device or pinned allocation failed */
-    TWDIC_EBOUNDS = 3,  /* a POI window/subset leaves the image or ROI */
+    TWDIC_EBOUNDS = 3,  /* a device buffer is smaller than the call needs (n entries,
+                           an image of stride x height); nothing was launched */
     TWDIC_EHIP = 4,     /* HIP runtime error (message in twdic_last_error) */
     TWDIC_EBUSY = 5,    /* serving queue full (admission refused) */
     TWDIC_ENODEV = 6    /* no usable GPU */
@@ -254,6 +255,17 @@ twdic_status twdic_get_buffer(twdic_ctx* ctx, int which, void** d_ptr, size_t* b
 twdic_status twdic_precompute_target(twdic_ctx* ctx);
 
 /* ---------------------------------------------------------- device memory */
+
+/* Bytes of memory from d_ptr to the end of the HIP allocation that holds it
+ * (device or pinned host memory); TWDIC_EINVAL when the runtime knows no
+ * allocation there.  No counterpart in the reference.  Every entry that takes
+ * device buffers checks each one this way before launching anything and
+ * returns TWDIC_EBOUNDS when one is shorter than the call needs (n entries, or
+ * stride x (h - 1) + w bytes of an image), TWDIC_EINVAL when one is not in a
+ * HIP allocation.  The check sees the allocation, not the caller's view of
+ * it: a buffer carved out of a larger allocation is held to that allocation's
+ * end. */
+twdic_status twdic_device_span(const void* d_ptr, size_t* bytes);
 
 twdic_status twdic_malloc(twdic_ctx* ctx, void** d_ptr, size_t bytes);
 twdic_status twdic_free(twdic_ctx* ctx, void* d_ptr);

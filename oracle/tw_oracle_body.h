@@ -211,6 +211,9 @@ static int SFX(gradients)(const uint8_t* img, int stride, int img_w, int img_h,
      * around the ROI must lie inside the image. */
     if (sx0 < 1 || sy0 < 1 || sx0 + rw + 1 > img_w || sy0 + rh + 1 > img_h) return -1;
 #define PIX(yy, xx) ((int)img[(size_t)(yy) * stride + (xx)])
+    /* rows in parallel, as GradientXY_2Images_m (TW_utils.cpp:445-519);
+     * every element is computed on its own, so the output is the same */
+#pragma omp parallel for schedule(static)
     for (int i = sy0; i < sy0 + rh; ++i)                    /* :260-273 */
         for (int j = sx0; j < sx0 + rw; ++j) {
             size_t o = (size_t)(i - sy0) * rw + (j - sx0);
@@ -249,6 +252,8 @@ static int SFX(bicubic_lut)(const uint8_t* img, int stride, int img_w, int img_h
     /* TW_utils.cpp:615-619 (the reference passes (cols, cols) as (W, H),
      * TW_paDIC_ICGN2D_CPU.cpp:317; the check below uses the true height). */
     if (img_h - (rh + sy0 + 1) < 0 || img_w - (rw + sx0 + 1) < 0) return -1;
+    /* rows in parallel, as BicubicCoefficients_m (TW_utils.cpp:789-888) */
+#pragma omp parallel for schedule(static)
     for (int i = 0; i < rh - 1; ++i)                        /* :643-690 */
         for (int j = 0; j < rw - 1; ++j) {
             REAL tao[16], alpha[16];
@@ -301,6 +306,7 @@ static int SFX(bspline_lut)(const uint8_t* img, int stride, int img_w, int img_h
             cp[k][l] = (REAL)SFX(bspline_cp)[k][l];
             bb[k][l] = (REAL)SFX(bspline_base)[k][l];
         }
+#pragma omp parallel for schedule(static)
     for (int i = 0; i < rh; ++i)
         for (int j = 0; j < rw; ++j) {
             REAL om[4][4], beta[4][4], c[4][4];

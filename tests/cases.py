@@ -79,3 +79,52 @@ def truth(case: Case, poi: np.ndarray) -> np.ndarray:
     for i, (y, x) in enumerate(poi):
         out[i] = api.displacement(case.deform, float(x), float(y))
     return out
+
+
+@dataclass
+class BatchPair:
+    """One image pair of the batched configuration: a size x size speckle pair
+    (0.015 spots/px^2, noise sigma 2) under a random affine field, subset
+    radius S, FFT-CC window N (32 for S <= 15, else 64), POIs on a `grid` px
+    grid from a border of max(N/2, S + 2 + ceil(8 + size/100)) px."""
+    size: int
+    S: int
+    grid: int
+    N: int
+    ref: np.ndarray
+    tar: np.ndarray
+    poi: np.ndarray
+    deform: api.Deform
+
+    def config(self, max_iter=20, delta_p=0.001) -> api.Config:
+        s = self.size
+        return api.make_config(s, s, 1, 1, s - 2, s - 2, self.S, self.S, 1, 1, 0, 0, self.N, max_iter, delta_p)
+
+    @property
+    def roi(self) -> tuple:
+        return (1, 1, self.size - 2, self.size - 2)
+
+
+def batch_pairs(count=64, seed=1003, on_pair=None) -> list[BatchPair]:
+    """The batched configuration: `count` speckle pairs with sizes drawn from
+    {512, 1024, 2048, 4096}^2, subset radius from {10, 15, 20, 30}, grid step
+    from {4, 5, 10}, strains uniform in [-1 %, 1 %] and translations uniform in
+    [-8, 8] px about the image centre; pair i's images from seed 1003000 + i.
+    on_pair(i, pair) is called after each pair is made (progress)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(count):
+        size = int(rng.choice([512, 1024, 2048, 4096]))
+        S = int(rng.choice([10, 15, 20, 30]))
+        g = int(rng.choice([4, 5, 10]))
+        N = 32 if S <= 15 else 64
+        d = api.affine(*rng.uniform(-8, 8, 2), *rng.uniform(-0.01, 0.01, 3), size / 2, size / 2)
+        ref, tar = api.synth_pair(size, size, int(0.015 * size * size), 1003000 + i, d, noise_sigma=2.0)
+        m = max(N // 2, S + 2 + int(np.ceil(8 + 0.01 * size)))
+        ys = np.arange(m, size - m, g)
+        yy, xx = np.meshgrid(ys, ys, indexing="ij")
+        poi = np.stack([yy.ravel(), xx.ravel()], 1).astype(np.int32)
+        out.append(BatchPair(size, S, g, N, ref, tar, poi, d))
+        if on_pair:
+            on_pair(i, out[-1])
+    return out

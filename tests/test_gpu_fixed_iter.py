@@ -1,0 +1,77 @@
+"""IC-GN parity at a fixed number of iterations: delta_p = 0 turns the loop
+test (TW_paDIC_ICGN2D_CPU.cpp:749-751) off, so the HIP path and both CPU
+oracles stop after exactly max_iter passes, and what differs between them is
+arithmetic alone, with no stopping-decision noise.
+
+On the 512^2 workload (all 2,304 POIs, 31 x 31 subsets) and a seeded sample of
+2,500 POIs plus the four corners of the 2048^2 workload (41 x 41 subsets),
+for max_iter in {1, 2, 3, 5}:
+  * every POI runs exactly max_iter iterations; flags equal both oracles;
+  * u, v within 1e-4 px of the FP32 oracle and of the FP64 oracle (the
+    parity bar for sub-pixel displacement);
+  * displacement gradients are reported against 1e-6 and held to
+    max(1e-6, 1.5x) of the reference FP32 path's own distance from FP64 at the
+    same iteration count (vs FP64) and max(1e-6, 2x) of it (vs FP32).  The
+    FP32 path's warped coordinates, ((x + W0 dx) + W1 dy) + W2 in FP32 as the
+    reference evaluates them, are quantised to half an ulp (1.2e-4 px at
+    x in [1024, 2048)), which moves its gradients up to 3.8e-6 from FP64 on
+    41 x 41 subsets, so a 1e-6 bar is below the reference's own rounding there.
+"""
+import numpy as np
+import pytest
+
+import oracle
+from tests import cases
+from tests.gpu_helpers import ctx_for, run_icgn
+
+pytestmark = pytest.mark.gpu
+
+GRAD = [1, 2, 4, 5]
+
+
+def _workload(name):
+    c = cases.speckle_512() if name == "C1" else cases.speckle_2048()
+    poi = c.poi()
+    n = len(poi)
+    if n > 5000:
+        rng = np.random.default_rng(2048)
+        side = int(round(np.sqrt(n)))
+        corners = np.array([0, side - 1, n - side, n - 1])
+        idx = np.unique(np.concatenate([rng.choice(n, 2500, replace=False), corners]))
+        poi = np.ascontiguousarray(poi[idx])
+    return c, poi
+
+
+def _dist(a, b, ok):
+    du = float(np.max(np.maximum(np.abs(a["u"] - b["u"]), np.abs(a["v"] - b["v"]))[ok]))
+    dg = float(np.max(np.abs(a["p6"][:, GRAD] - b["p6"][:, GRAD])[ok]))
+    return du, dg
+
+
+@pytest.mark.parametrize("name", ["C1", "C2"])
+def test_fixed_iteration_parity(name):
+    c, poi = _workload(name)
+    ex = oracle.fftcc_exact(c.ref, c.tar, poi, c.N)
+    u0, v0 = ex["u"].astype(np.float32), ex["v"].astype(np.float32)
+    rows = []
+    for k in (1, 2, 3, 5):
+        print(f"[gpu-test] fixed iterations {name} max_iter={k}: {len(poi)} POIs", flush=True)
+        with ctx_for(c, max_iter=k, delta_p=0.0) as ctx:
+            g = run_icgn(ctx, poi, u0, v0)
+        o = oracle.icgn(c.ref, c.tar, c.roi, c.S, c.S, poi, u0, v0, max_iter=k, delta_p=0.0)
+        o64 = oracle.icgn(c.ref, c.tar, c.roi, c.S, c.S, poi, u0.astype(np.float64), v0.astype(np.float64),
+                          max_iter=k, delta_p=0.0, precision="f64")
+        assert np.array_equal(g["flags"], o["flags"]) and np.array_equal(g["flags"], o64["flags"])
+        ok = g["flags"] == oracle.SUCCESS
+        assert ok.mean() > 0.99
+        assert np.all(g["iters"][ok] == k) and np.all(o["iters"][ok] == k) and np.all(o64["iters"][ok] == k)
+        u32, g32 = _dist(g, o, ok)
+        u64, g64 = _dist(g, o64, ok)
+        uown, gown = _dist(o, o64, ok)
+        rows.append((k, u32, u64, uown, g32, g64, gown))
+        print(f"  max_iter {k}: u,v vs FP32 {u32:.3g} px, vs FP64 {u64:.3g} px (FP32 path vs FP64 {uown:.3g}); "
+              f"gradients vs FP32 {g32:.3g}, vs FP64 {g64:.3g} (FP32 path vs FP64 {gown:.3g}; bar 1e-6)",
+              flush=True)
+        assert u32 <= 1e-4 and u64 <= 1e-4, (u32, u64)
+        assert g64 <= max(1e-6, 1.5 * gown), (g64, gown)
+        assert g32 <= max(1e-6, 2.0 * gown), (g32, gown)

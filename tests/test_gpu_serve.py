@@ -157,3 +157,68 @@ def test_memory_admission_and_dropping(world):
     for k, _, out in res:
         if k % 4 == res[0][0] % 4:
             _same(out, ref_out)
+
+
+def test_mix_sampled_against_oracle(world):
+    """60 concurrent requests of every class, both references and several
+    targets; a seeded sample of 24 POIs of every request is checked against
+    the oracles (FFT-CC ZNCC within 1e-6 of the exact-integer value; IC-GN
+    from the exact integer guesses with the bars of the 2048^2 workload test,
+    scaled by the reference FP32 path's own distance from FP64 on the same
+    POIs)."""
+    from tests.gpu_helpers import check_icgn
+    rng = np.random.default_rng(11)
+    with api.Server(slots=8, queue_capacity=512, max_width=W, max_height=H, max_pois=20000) as srv:
+        srv.set_reference(0, world["refs"][0])
+        srv.set_reference(1, world["refs"][1])
+        ids = [srv.add_poiset(p) for p in world["sets"]]
+        reqs = []
+        for k in range(60):
+            r = int(rng.integers(0, 2))
+            c = int(rng.integers(0, 3))
+            si = int(rng.integers(0, 4))
+            ti = int(rng.integers(0, 3))
+            S, N = CLASSES[c]
+            t, out = srv.submit(r, ids[si], world["tars"][r][ti], S, N)
+            reqs.append((t, out, r, si, ti, c))
+        for t, *_ in reqs:
+            srv.wait(t)
+        st = srv.stats()
+        assert st.completed == 60 and st.failed == 0 and st.slots_busy_peak > 1
+    # group the samples by (reference, target, class): one oracle run each
+    groups = {}
+    for t, out, r, si, ti, c in reqs:
+        poi = world["sets"][si]
+        idx = rng.choice(len(poi), 24, replace=False)
+        groups.setdefault((r, ti, c), []).append((out, poi, idx))
+    g_s, o_s, o64_s = [], [], []
+    for (r, ti, c), members in groups.items():
+        S, N = CLASSES[c]
+        ref, tar = world["refs"][r], world["tars"][r][ti]
+        sp = np.ascontiguousarray(np.concatenate([poi[idx] for _, poi, idx in members]))
+        ex = oracle.fftcc_exact(ref, tar, sp, N)
+        zn = np.concatenate([out["zncc"][idx] for out, _, idx in members])
+        assert np.max(np.abs(zn - ex["zncc"])) < 1e-6, (r, ti, c)
+        u0, v0 = ex["u"].astype(np.float32), ex["v"].astype(np.float32)
+        o_s.append(oracle.icgn(ref, tar, (1, 1, W - 2, H - 2), S, S, sp, u0, v0))
+        o64_s.append(oracle.icgn(ref, tar, (1, 1, W - 2, H - 2), S, S, sp, u0.astype(np.float64),
+                                 v0.astype(np.float64), precision="f64"))
+        g_s.append({k: np.concatenate([out[k][idx] for out, _, idx in members])
+                    for k in ("u", "v", "p6", "iters", "flags")})
+
+    def cat(rs):
+        return {k: np.concatenate([x[k] for x in rs]) for k in rs[0] if k != "times_ms"}
+    g, o, o64 = cat(g_s), cat(o_s), cat(o64_s)
+    assert np.mean(g["flags"] == 0) > 0.95
+    same = (o["iters"] == o64["iters"]) & (o["flags"] == 0)
+    own_u = float(np.max(np.maximum(np.abs(o["u"] - o64["u"]), np.abs(o["v"] - o64["v"]))[same]))
+    own_g = float(np.max(np.abs(o["p6"][:, [1, 2, 4, 5]] - o64["p6"][:, [1, 2, 4, 5]])[same]))
+    own = int(np.count_nonzero(o["iters"] != o64["iters"]))
+    fl = np.nonzero(o["iters"] != o64["iters"])[0]
+    stop = np.where(o["iters"][fl] < o64["iters"][fl], o["dpnorm"][fl], o64["dpnorm"][fl])
+    band = max(0.05, 1.25 * float(np.max(np.abs(stop / 1e-3 - 1.0)))) if len(fl) else 0.05
+    cap = max(2, int(0.002 * len(g["u"])), 3 * own + 2)
+    print(f"  {len(g['u'])} sampled POIs in {len(groups)} (reference, target, class) groups; FP32 oracle vs FP64: "
+          f"u, v {own_u:.3g} px, gradients {own_g:.3g}, iteration flips {own}")
+    check_icgn(g, o64, utol=max(5e-5, 1.25 * own_u), gtol=max(2e-6, 1.5 * own_g), band=band, max_flips=cap)
+    check_icgn(g, o, utol=max(1e-4, 2 * own_u), gtol=max(3e-6, 2 * own_g), band=band, max_flips=cap)

@@ -107,6 +107,7 @@ _SIGS = {
     "twdic_reset_poi_state": ([_VP], _I),
     "twdic_get_buffer": ([_VP, _I, C.POINTER(_VP), C.POINTER(_SZ)], _I),
     "twdic_precompute_target": ([_VP], _I),
+    "twdic_device_span": ([_VP, C.POINTER(_SZ)], _I),
     "twdic_malloc": ([_VP, C.POINTER(_VP), _SZ], _I),
     "twdic_free": ([_VP, _VP], _I),
     "twdic_memcpy": ([_VP, _VP, _VP, _SZ, _I], _I),
@@ -174,6 +175,16 @@ def exported_symbols() -> list[str]:
 
 def _ptr(a: np.ndarray) -> int:
     return a.ctypes.data
+
+
+def device_span(ptr: int) -> int:
+    """Bytes from a device pointer to the end of the HIP allocation holding it
+    (twdic_device_span); raises when the runtime knows no allocation there."""
+    n = C.c_size_t()
+    st = lib().twdic_device_span(C.c_void_p(ptr), C.byref(n))
+    if st != OK:
+        raise TwdicError(f"twdic_device_span: status {st}")
+    return n.value
 
 
 # --------------------------------------------------------------------------- synth

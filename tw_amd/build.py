@@ -73,7 +73,8 @@ def build(verbose: bool = False) -> Path:
     _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", str(tmp), *objs, "-lgomp"])
     os.replace(tmp, LIB)
     # and TW_Core-style worker drivers, linked against the library in-tree
-    exes = [(OUT_DIR / "tw_frame_loop", "frame_loop.cpp"), (OUT_DIR / "tw_icgn_worker", "icgn_worker.cpp")]
+    exes = [(OUT_DIR / "tw_frame_loop", "frame_loop.cpp"), (OUT_DIR / "tw_icgn_worker", "icgn_worker.cpp"),
+            (OUT_DIR / "tw_fftcc_host", "fftcc_host.cpp")]
     for exe, src in exes:
         _run([HIPCC, "-O2", "-std=c++17", f"-I{INCLUDE}", str(CPPSRC / "examples" / src),
               "-o", str(exe), f"-L{OUT_DIR}", "-ltwdic", "-Wl,-rpath,$ORIGIN"])

@@ -110,6 +110,7 @@ void release(twdic_batch* b)
         if (p) (void)hipFree(p);
     for (int f = 0; f < 4; ++f) { b->d_mitems[f] = nullptr; b->nm[f] = 0; }
     for (int t = 0; t < 2; ++t) { b->d_wg[t] = nullptr; b->nwg[t] = 0; }
+    for (int c = 0; c < 3; ++c) b->nf[c] = 0;
     for (auto& kv : b->tw) (void)hipFree(kv.second);
     b->tw.clear();
     b->d_desc = nullptr; b->d_items = nullptr;
@@ -131,6 +132,43 @@ const char* check_pair(const twdic_pair& p)
     if (p.fft_n < 2 || p.fft_n > kMaxFftN || (p.fft_n & 1)) return "fft_n must be even and in [2, 64]";
     if (p.max_iter < 1 || !(p.delta_p >= 0.f) || p.n < 0) return "bad iteration parameters or POI count";
     return nullptr;
+}
+
+// Every device buffer of a pair must lie in a HIP allocation that holds what
+// the pair's kernels touch (twdic_device_span); checked before anything runs.
+twdic_status check_pair_spans(twdic_batch* b, int i, const twdic_pair& p)
+{
+    const size_t n = (size_t)p.n, img = (size_t)p.stride * (p.img_h - 1) + p.img_w;
+    const struct {
+        const void* ptr;
+        size_t bytes;
+        const char* name;
+    } need[] = {{p.d_ref, img, "d_ref"}, {p.d_tar, img, "d_tar"},      {p.d_poi_yx, 8 * n, "d_poi_yx"},
+                {p.d_u, 4 * n, "d_u"},   {p.d_v, 4 * n, "d_v"},        {p.d_zncc, 4 * n, "d_zncc"},
+                {p.d_p6, 24 * n, "d_p6"}, {p.d_iters, 4 * n, "d_iters"}, {p.d_flags, n, "d_flags"}};
+    for (const auto& k : need) {
+        if (!k.ptr || !k.bytes) continue;
+        size_t have = 0;
+        if (twdic_device_span(k.ptr, &have) != TWDIC_OK)
+            return bfail(b, TWDIC_EINVAL, "pair " + std::to_string(i) + ": " + k.name + " is not in a device allocation");
+        if (have < k.bytes)
+            return bfail(b, TWDIC_EBOUNDS, "pair " + std::to_string(i) + ": " + k.name + " holds " +
+                                               std::to_string(have) + " bytes, the pair needs " +
+                                               std::to_string(k.bytes));
+    }
+    return TWDIC_OK;
+}
+
+// Forgets the pairs after a failed set_pairs, so no later call launches on
+// item lists or descriptors that were only partly built.
+void clear_pairs(twdic_batch* b)
+{
+    release(b);
+    b->pairs.clear();
+    b->h_desc.clear();
+    b->total = 0;
+    b->state_valid = false;
+    b->graph_dirty = true;
 }
 
 twdic_status upload_desc(twdic_batch* b)
@@ -276,69 +314,10 @@ std::string signature(twdic_batch* b)
 
 }  // namespace
 
-extern "C" {
+namespace {
 
-twdic_status twdic_batch_create(int device, twdic_batch** out)
+twdic_status set_pairs_impl(twdic_batch* b, const twdic_pair* pairs, int npairs)
 {
-    if (!out) return TWDIC_EINVAL;
-    *out = nullptr;
-    int ndev = 0;
-    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0 || device < 0 || device >= ndev) return TWDIC_ENODEV;
-    twdic_batch* b = new (std::nothrow) twdic_batch();
-    if (!b) return TWDIC_ENOMEM;
-    b->device = device;
-    if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) {
-        delete b;
-        return TWDIC_EHIP;
-    }
-    for (auto& e : b->ev)
-        if (hipEventCreate(&e) != hipSuccess) {
-            twdic_batch_destroy(b);
-            return TWDIC_EHIP;
-        }
-    bool ok = hipEventCreateWithFlags(&b->fork, hipEventDisableTiming) == hipSuccess;
-    for (int k = 0; k < twdic_batch::kLanes && ok; ++k)
-        ok = hipStreamCreateWithFlags(&b->lane[k], hipStreamNonBlocking) == hipSuccess &&
-             hipEventCreateWithFlags(&b->join[k], hipEventDisableTiming) == hipSuccess;
-    if (!ok) {
-        twdic_batch_destroy(b);
-        return TWDIC_EHIP;
-    }
-    if (const char* v = std::getenv("TWDIC_BATCH_STREAMS")) b->lanes = std::atoi(v);
-    if (const char* v = std::getenv("TWDIC_BATCH_FORM")) b->flat = std::strcmp(v, "direct") != 0;
-    *out = b;
-    return TWDIC_OK;
-}
-
-twdic_status twdic_batch_destroy(twdic_batch* b)
-{
-    if (!b) return TWDIC_EINVAL;
-    (void)hipSetDevice(b->device);
-    for (auto& l : b->lane)
-        if (l) (void)hipStreamSynchronize(l);
-    release(b);
-    for (auto& e : b->ev)
-        if (e) (void)hipEventDestroy(e);
-    if (b->fork) (void)hipEventDestroy(b->fork);
-    for (int k = 0; k < twdic_batch::kLanes; ++k) {
-        if (b->join[k]) (void)hipEventDestroy(b->join[k]);
-        if (b->lane[k]) (void)hipStreamDestroy(b->lane[k]);
-    }
-    if (b->stream) (void)hipStreamDestroy(b->stream);
-    delete b;
-    return TWDIC_OK;
-}
-
-const char* twdic_batch_last_error(const twdic_batch* b) { return b ? b->err.c_str() : "null batch"; }
-void* twdic_batch_stream(twdic_batch* b) { return b ? (void*)b->stream : nullptr; }
-
-twdic_status twdic_batch_set_pairs(twdic_batch* b, const twdic_pair* pairs, int npairs)
-{
-    if (!b || npairs < 0 || (npairs > 0 && !pairs)) return TWDIC_EINVAL;
-    for (int i = 0; i < npairs; ++i)
-        if (const char* why = check_pair(pairs[i]))
-            return bfail(b, TWDIC_EINVAL, "pair " + std::to_string(i) + ": " + why);
-    (void)hipSetDevice(b->device);
     release(b);
     b->pairs.assign(pairs, pairs + npairs);
     b->h_desc.assign(npairs, PairDesc{});
@@ -434,9 +413,91 @@ twdic_status twdic_batch_set_pairs(twdic_batch* b, const twdic_pair* pairs, int 
     return upload_desc(b);
 }
 
+}  // namespace
+
+extern "C" {
+
+twdic_status twdic_batch_create(int device, twdic_batch** out)
+{
+    if (!out) return TWDIC_EINVAL;
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0 || device < 0 || device >= ndev) return TWDIC_ENODEV;
+    twdic_batch* b = new (std::nothrow) twdic_batch();
+    if (!b) return TWDIC_ENOMEM;
+    b->device = device;
+    if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete b;
+        return TWDIC_EHIP;
+    }
+    for (auto& e : b->ev)
+        if (hipEventCreate(&e) != hipSuccess) {
+            twdic_batch_destroy(b);
+            return TWDIC_EHIP;
+        }
+    bool ok = hipEventCreateWithFlags(&b->fork, hipEventDisableTiming) == hipSuccess;
+    for (int k = 0; k < twdic_batch::kLanes && ok; ++k)
+        ok = hipStreamCreateWithFlags(&b->lane[k], hipStreamNonBlocking) == hipSuccess &&
+             hipEventCreateWithFlags(&b->join[k], hipEventDisableTiming) == hipSuccess;
+    if (!ok) {
+        twdic_batch_destroy(b);
+        return TWDIC_EHIP;
+    }
+    if (const char* v = std::getenv("TWDIC_BATCH_STREAMS")) b->lanes = std::atoi(v);
+    if (const char* v = std::getenv("TWDIC_BATCH_FORM")) b->flat = std::strcmp(v, "direct") != 0;
+    *out = b;
+    return TWDIC_OK;
+}
+
+twdic_status twdic_batch_destroy(twdic_batch* b)
+{
+    if (!b) return TWDIC_EINVAL;
+    (void)hipSetDevice(b->device);
+    for (auto& l : b->lane)
+        if (l) (void)hipStreamSynchronize(l);
+    release(b);
+    for (auto& e : b->ev)
+        if (e) (void)hipEventDestroy(e);
+    if (b->fork) (void)hipEventDestroy(b->fork);
+    for (int k = 0; k < twdic_batch::kLanes; ++k) {
+        if (b->join[k]) (void)hipEventDestroy(b->join[k]);
+        if (b->lane[k]) (void)hipStreamDestroy(b->lane[k]);
+    }
+    if (b->stream) (void)hipStreamDestroy(b->stream);
+    delete b;
+    return TWDIC_OK;
+}
+
+const char* twdic_batch_last_error(const twdic_batch* b) { return b ? b->err.c_str() : "null batch"; }
+void* twdic_batch_stream(twdic_batch* b) { return b ? (void*)b->stream : nullptr; }
+
+twdic_status twdic_batch_set_pairs(twdic_batch* b, const twdic_pair* pairs, int npairs)
+{
+    if (!b || npairs < 0 || (npairs > 0 && !pairs)) return TWDIC_EINVAL;
+    for (int i = 0; i < npairs; ++i)
+        if (const char* why = check_pair(pairs[i]))
+            return bfail(b, TWDIC_EINVAL, "pair " + std::to_string(i) + ": " + why);
+    (void)hipSetDevice(b->device);
+    for (int i = 0; i < npairs; ++i) {
+        twdic_status st = check_pair_spans(b, i, pairs[i]);
+        if (st != TWDIC_OK) return st;
+    }
+    const twdic_status st = set_pairs_impl(b, pairs, npairs);
+    if (st != TWDIC_OK) clear_pairs(b);
+    return st;
+}
+
+
 twdic_status twdic_batch_set_targets(twdic_batch* b, const uint8_t* const* d_tars)
 {
     if (!b || !d_tars) return TWDIC_EINVAL;
+    for (size_t i = 0; i < b->pairs.size(); ++i) {
+        if (!d_tars[i]) continue;
+        twdic_pair q = b->pairs[i];
+        q.d_tar = d_tars[i];
+        twdic_status st = check_pair_spans(b, (int)i, q);
+        if (st != TWDIC_OK) return st;
+    }
     for (size_t i = 0; i < b->pairs.size(); ++i) {
         if (!d_tars[i]) continue;
         b->pairs[i].d_tar = d_tars[i];
@@ -449,6 +510,13 @@ twdic_status twdic_batch_set_targets(twdic_batch* b, const uint8_t* const* d_tar
 twdic_status twdic_batch_set_references(twdic_batch* b, const uint8_t* const* d_refs)
 {
     if (!b || !d_refs) return TWDIC_EINVAL;
+    for (size_t i = 0; i < b->pairs.size(); ++i) {
+        if (!d_refs[i]) continue;
+        twdic_pair q = b->pairs[i];
+        q.d_ref = d_refs[i];
+        twdic_status st = check_pair_spans(b, (int)i, q);
+        if (st != TWDIC_OK) return st;
+    }
     for (size_t i = 0; i < b->pairs.size(); ++i) {
         if (!d_refs[i]) continue;
         b->pairs[i].d_ref = d_refs[i];

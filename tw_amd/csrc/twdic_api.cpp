@@ -7,7 +7,9 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <initializer_list>
 #include <new>
+#include <string>
 
 #include "twdic_internal.h"
 
@@ -35,6 +37,56 @@ twdic_status hip_fail(twdic_ctx* c, hipError_t e, const char* where)
     do {                                                          \
         hipError_t e_ = (expr);                                   \
         if (e_ != hipSuccess) return hip_fail((ctx), e_, #expr);  \
+    } while (0)
+
+// Bytes from p to the end of the allocation that holds it; false when the
+// runtime knows no allocation at p (the failed query's error is cleared so a
+// later launch check does not report it).
+bool device_span(const void* p, size_t* bytes)
+{
+    hipDeviceptr_t base = nullptr;
+    size_t size = 0;
+    if (hipMemGetAddressRange(&base, &size, const_cast<void*>(p)) != hipSuccess || !base) {
+        (void)hipGetLastError();
+        return false;
+    }
+    const uintptr_t b = reinterpret_cast<uintptr_t>(base), q = reinterpret_cast<uintptr_t>(p);
+    if (q < b || q - b > size) return false;
+    *bytes = size - (q - b);
+    return true;
+}
+
+struct Need {
+    const void* p;        // null: not given (optional output), not checked
+    size_t bytes;
+    const char* name;
+};
+
+// Every device buffer a call names must lie in an allocation that holds the
+// bytes the call touches; checked before anything is launched.
+twdic_status check_spans(twdic_ctx* c, const char* call, std::initializer_list<Need> needs)
+{
+    for (const Need& k : needs) {
+        if (!k.p || !k.bytes) continue;
+        size_t have = 0;
+        if (!device_span(k.p, &have)) {
+            if (c) c->err = std::string(call) + ": " + k.name + " is not in a device allocation";
+            return TWDIC_EINVAL;
+        }
+        if (have < k.bytes) {
+            if (c)
+                c->err = std::string(call) + ": " + k.name + " holds " + std::to_string(have) + " bytes, the call needs " +
+                         std::to_string(k.bytes);
+            return TWDIC_EBOUNDS;
+        }
+    }
+    return TWDIC_OK;
+}
+
+#define TW_SPANS(ctx, call, ...)                                           \
+    do {                                                                   \
+        twdic_status s_ = check_spans((ctx), (call), {__VA_ARGS__});       \
+        if (s_ != TWDIC_OK) return s_;                                     \
     } while (0)
 
 // Fftcc2D::recomputeNumPOI (TW_paDIC_FFTCC2D.cpp:88-94): float division,
@@ -76,6 +128,7 @@ twdic_status upload(twdic_ctx* c, uint8_t* dst, const uint8_t* img, int stride, 
     const twdic_config& g = c->cfg;
     if (!img || stride < g.img_w) return fail(c, TWDIC_EINVAL, "image pointer/stride invalid");
     if (on_device) {
+        TW_SPANS(c, "image upload", {img, (size_t)stride * (g.img_h - 1) + g.img_w, "device image"});
         TW_HIP(c, hipMemcpy2DAsync(dst, c->dstride, img, stride, g.img_w, g.img_h, hipMemcpyDeviceToDevice, c->stream));
         return TWDIC_OK;
     }
@@ -226,6 +279,12 @@ void drop_graph(twdic_ctx* c)
 
 extern "C" {
 
+twdic_status twdic_device_span(const void* d_ptr, size_t* bytes)
+{
+    if (!d_ptr || !bytes) return TWDIC_EINVAL;
+    return device_span(d_ptr, bytes) ? TWDIC_OK : TWDIC_EINVAL;
+}
+
 twdic_status twdic_num_poi(const twdic_config* cfg, int32_t* nx, int32_t* ny)
 {
     if (!cfg || !nx || !ny) return TWDIC_EINVAL;
@@ -362,6 +421,7 @@ twdic_status twdic_poi_grid(twdic_ctx* c, int32_t* d_poi)
 {
     if (!c || !d_poi) return TWDIC_EINVAL;
     const twdic_config& g = c->cfg;
+    TW_SPANS(c, "twdic_poi_grid", {d_poi, 8 * (size_t)c->nx * c->ny, "d_poi"});
     TW_HIP(c, launch_poi_grid(d_poi, g.roi_x, g.roi_y, c->nx, c->ny, g.mx, g.my, g.sx, g.sy, g.gx, g.gy, c->stream));
     return TWDIC_OK;
 }
@@ -371,6 +431,9 @@ twdic_status twdic_fftcc(twdic_ctx* c, const int32_t* d_poi, int n, float* d_u, 
 {
     if (!c || n < 0 || (n > 0 && (!d_poi || !d_u || !d_v || !d_zncc))) return fail(c, TWDIC_EINVAL, "twdic_fftcc: bad arguments");
     if (!c->ref_loaded || !c->tar_loaded) return fail(c, TWDIC_EINVAL, "twdic_fftcc: images not set");
+    const size_t m = (size_t)n;
+    TW_SPANS(c, "twdic_fftcc", {d_poi, 8 * m, "d_poi"}, {d_u, 4 * m, "d_u"}, {d_v, 4 * m, "d_v"},
+             {d_zncc, 4 * m, "d_zncc"}, {d_status, m, "d_status"});
     TW_HIP(c, launch_fftcc(fftcc_params(c, d_poi, n, d_u, d_v, d_zncc, d_status), c->fftcc_variant, c->stream));
     return TWDIC_OK;
 }
@@ -398,6 +461,10 @@ twdic_status twdic_icgn(twdic_ctx* c, const int32_t* d_poi, int n, float* d_u, f
     if (!c || n < 0 || (n > 0 && (!d_poi || !d_u || !d_v))) return fail(c, TWDIC_EINVAL, "twdic_icgn: bad arguments");
     if (!c->ref_loaded || !c->tar_loaded) return fail(c, TWDIC_EINVAL, "twdic_icgn: images not set");
     if (n == 0) return TWDIC_OK;
+    const size_t m = (size_t)n;
+    TW_SPANS(c, "twdic_icgn", {d_poi, 8 * m, "d_poi"}, {d_u, 4 * m, "d_u"}, {d_v, 4 * m, "d_v"},
+             {d_p6, 24 * m, "d_p6"}, {d_iters, 4 * m, "d_iters"}, {d_flags, m, "d_flags"},
+             {d_dpnorm, 4 * m, "d_dpnorm"});
     IcgnParams p;
     twdic_status st = icgn_prepare(c, d_poi, n, d_u, d_v, d_p6, d_iters, d_flags, d_dpnorm, p);
     if (st != TWDIC_OK) return st;
@@ -418,6 +485,9 @@ twdic_status twdic_frame(twdic_ctx* c, const int32_t* d_poi, int n, float* d_u, 
     if (!c || n < 0 || (n > 0 && (!d_poi || !d_u || !d_v || !d_zncc))) return fail(c, TWDIC_EINVAL, "twdic_frame: bad arguments");
     if (!c->ref_loaded || !c->tar_loaded) return fail(c, TWDIC_EINVAL, "twdic_frame: images not set");
     if (n == 0) return TWDIC_OK;
+    const size_t m = (size_t)n;
+    TW_SPANS(c, "twdic_frame", {d_poi, 8 * m, "d_poi"}, {d_u, 4 * m, "d_u"}, {d_v, 4 * m, "d_v"},
+             {d_zncc, 4 * m, "d_zncc"}, {d_p6, 24 * m, "d_p6"}, {d_iters, 4 * m, "d_iters"}, {d_flags, m, "d_flags"});
     const int chunks = n >= 4096 * c->frame_chunks ? c->frame_chunks : 1;
     if (chunks == 1) {
         twdic_status st = twdic_fftcc(c, d_poi, n, d_u, d_v, d_zncc, nullptr);
@@ -451,6 +521,9 @@ twdic_status twdic_frame_capture(twdic_ctx* c, const int32_t* d_poi, int n, floa
 {
     if (!c || n <= 0 || !d_poi || !d_u || !d_v || !d_zncc) return fail(c, TWDIC_EINVAL, "twdic_frame_capture: bad arguments");
     if (!c->ref_loaded || !c->tar_loaded) return fail(c, TWDIC_EINVAL, "twdic_frame_capture: images not set");
+    const size_t m = (size_t)n;
+    TW_SPANS(c, "twdic_frame_capture", {d_poi, 8 * m, "d_poi"}, {d_u, 4 * m, "d_u"}, {d_v, 4 * m, "d_v"},
+             {d_zncc, 4 * m, "d_zncc"}, {d_p6, 24 * m, "d_p6"}, {d_iters, 4 * m, "d_iters"}, {d_flags, m, "d_flags"});
     // reference gradients and per-POI state now, outside the graph
     IcgnParams ip;
     twdic_status st = icgn_prepare(c, d_poi, n, d_u, d_v, d_p6, d_iters, d_flags, nullptr, ip);
@@ -545,6 +618,8 @@ static twdic_status track_finish(twdic_ctx* c, hipError_t e)
 twdic_status twdic_track_update_poi(twdic_ctx* c, int32_t* d_poi, int n, const float* d_u, const float* d_v)
 {
     if (n < 0 || (n > 0 && (!d_poi || !d_u || !d_v))) return c ? fail(c, TWDIC_EINVAL, "track_update_poi: bad arguments") : TWDIC_EINVAL;
+    TW_SPANS(c, "track_update_poi", {d_poi, 8 * (size_t)n, "d_poi"}, {d_u, 4 * (size_t)n, "d_u"},
+             {d_v, 4 * (size_t)n, "d_v"});
     if (c && c->refstate_poi == d_poi) c->refstate_version = ~0ull;     // the POIs moved
     return track_finish(c, launch_track_update_poi(d_poi, n, d_u, d_v, c ? c->stream : nullptr));
 }
@@ -554,6 +629,8 @@ twdic_status twdic_track_current_poi(twdic_ctx* c, const int32_t* d_poi, int n, 
 {
     if (n < 0 || (n > 0 && (!d_poi || !d_u || !d_v || !d_out)))
         return c ? fail(c, TWDIC_EINVAL, "track_current_poi: bad arguments") : TWDIC_EINVAL;
+    TW_SPANS(c, "track_current_poi", {d_poi, 8 * (size_t)n, "d_poi"}, {d_u, 4 * (size_t)n, "d_u"},
+             {d_v, 4 * (size_t)n, "d_v"}, {d_out, 8 * (size_t)n, "d_out"});
     return track_finish(c, launch_track_current_poi(d_poi, n, d_u, d_v, d_out, c ? c->stream : nullptr));
 }
 
@@ -562,6 +639,8 @@ twdic_status twdic_track_accumulate_uv(twdic_ctx* c, int n, const float* d_u, co
 {
     if (n < 0 || (n > 0 && (!d_u || !d_v || !d_acc_u || !d_acc_v)))
         return c ? fail(c, TWDIC_EINVAL, "track_accumulate_uv: bad arguments") : TWDIC_EINVAL;
+    TW_SPANS(c, "track_accumulate_uv", {d_u, 4 * (size_t)n, "d_u"}, {d_v, 4 * (size_t)n, "d_v"},
+             {d_acc_u, 4 * (size_t)n, "d_acc_u"}, {d_acc_v, 4 * (size_t)n, "d_acc_v"});
     return track_finish(c, launch_track_accumulate_uv(n, d_u, d_v, d_acc_u, d_acc_v, c ? c->stream : nullptr));
 }
 
@@ -587,6 +666,9 @@ twdic_status twdic_strain_ls(twdic_ctx* c, const float* d_field, int nx, int ny,
 {
     if (!c || !d_field || nx < 1 || ny < 1 || grid < 1 || M < 1 || 2 * M + 1 > nx || 2 * M + 1 > ny)
         return c ? fail(c, TWDIC_EINVAL, "strain_ls: bad arguments") : TWDIC_EINVAL;
+    const size_t no = (size_t)(nx - 2 * M) * (ny - 2 * M);
+    TW_SPANS(c, "strain_ls", {d_field, 4 * (size_t)nx * ny, "d_field"}, {d_dfdx, 8 * no, "d_dfdx"},
+             {d_dfdy, 8 * no, "d_dfdy"});
     TW_HIP(c, launch_strain_ls(d_field, nx, ny, grid, M, d_dfdx, d_dfdy, c->stream));
     return TWDIC_OK;
 }
@@ -615,6 +697,7 @@ twdic_status twdic_memcpy(twdic_ctx* c, void* dst, const void* src, size_t bytes
     if (!bytes) return TWDIC_OK;
     hipMemcpyKind k = kind == 0 ? hipMemcpyHostToDevice : kind == 1 ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice;
     if (kind < 0 || kind > 2) return fail(c, TWDIC_EINVAL, "memcpy kind");
+    TW_SPANS(c, "twdic_memcpy", {kind != 1 ? dst : nullptr, bytes, "dst"}, {kind != 0 ? src : nullptr, bytes, "src"});
     TW_HIP(c, hipMemcpyAsync(dst, src, bytes, k, c->stream));
     if (kind != 2) TW_HIP(c, hipStreamSynchronize(c->stream));
     return TWDIC_OK;
@@ -637,6 +720,7 @@ twdic_status twdic_memset(twdic_ctx* c, void* d_ptr, int value, size_t bytes)
 {
     if (!c || (bytes && !d_ptr)) return TWDIC_EINVAL;
     if (!bytes) return TWDIC_OK;
+    TW_SPANS(c, "twdic_memset", {d_ptr, bytes, "d_ptr"});
     TW_HIP(c, hipMemsetAsync(d_ptr, value, bytes, c->stream));
     return TWDIC_OK;
 }
