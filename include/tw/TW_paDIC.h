@@ -44,10 +44,12 @@ struct GPUHandle_ICGN {
     real_t* m_d_fV = nullptr;
     real_t* m_d_fRx = nullptr;                 // ROI-relative fp32 gradients of the reference
     real_t* m_d_fRy = nullptr;
-    real_t* m_d_fTx = nullptr;                 // target gradients (filled by the LUT interpolation mode)
+    real_t* m_d_fTx = nullptr;                 // target gradients: written only by twdic_precompute_target on the
+                                               // context (inspection); IC-GN never reads them
     real_t* m_d_fTy = nullptr;
     real_t* m_d_fTxy = nullptr;
-    real_t4* m_d_f4InterpolationLUT = nullptr; // [roi_h][roi_w][16] fp32, element 4k+l ~ y^k x^l
+    real_t4* m_d_f4InterpolationLUT = nullptr; // [roi_h][roi_w][16] fp32, element 4k+l ~ y^k x^l; written per
+                                               // frame in the LUT and B-spline sources, unused by the fused one
     int* m_d_iIterationNums = nullptr;         // 1-based iteration count per POI
     real_t* m_d_fSubsetR = nullptr;            // not materialised
     real_t* m_d_fSubsetT = nullptr;

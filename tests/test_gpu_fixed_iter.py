@@ -7,15 +7,25 @@ On the 512^2 workload (all 2,304 POIs, 31 x 31 subsets) and a seeded sample of
 2,500 POIs plus the four corners of the 2048^2 workload (41 x 41 subsets),
 for max_iter in {1, 2, 3, 5}:
   * every POI runs exactly max_iter iterations; flags equal both oracles;
-  * u, v within 1e-4 px of the FP32 oracle and of the FP64 oracle (the
-    parity bar for sub-pixel displacement);
+  * u, v within 1e-4 px of the FP64 oracle (the parity bar for sub-pixel
+    displacement), and within max(1e-4, 1.5 ulp of the largest coordinate)
+    of the FP32 oracle.  Both FP32 paths evaluate the warped coordinates
+    ((x + W0 dx) + W1 dy) + W2 in FP32, as the reference does, so those are
+    quantised to the FP32 spacing at the POI coordinates (1.22e-4 px for x in
+    [1024, 2048)).  That quantisation is most of the FP32 path's distance from
+    FP64 (an FP64 oracle that rounds only its coordinates to FP32 sits as far
+    from FP64: 8.1e-5 px at max_iter 2 on the 2048^2 sample), and it is a
+    step function of the warp: two FP32 paths whose warps differ in the last
+    bit after the first update can land a pixel one spacing apart, so they
+    may differ by about one spacing even at equal iteration counts (that
+    FP64-with-FP32-coordinates oracle and the FP32 oracle differ by 1.26e-4 px
+    there, median 7e-7);
   * displacement gradients are reported against 1e-6 and held to
     max(1e-6, 1.5x) of the reference FP32 path's own distance from FP64 at the
     same iteration count (vs FP64) and max(1e-6, 2x) of it (vs FP32).  The
-    FP32 path's warped coordinates, ((x + W0 dx) + W1 dy) + W2 in FP32 as the
-    reference evaluates them, are quantised to half an ulp (1.2e-4 px at
-    x in [1024, 2048)), which moves its gradients up to 3.8e-6 from FP64 on
-    41 x 41 subsets, so a 1e-6 bar is below the reference's own rounding there.
+    same coordinate quantisation moves the FP32 path's gradients up to 3.8e-6
+    from FP64 on 41 x 41 subsets, so a 1e-6 bar is below the reference's own
+    rounding there.
 """
 import numpy as np
 import pytest
@@ -72,6 +82,8 @@ def test_fixed_iteration_parity(name):
         print(f"  max_iter {k}: u,v vs FP32 {u32:.3g} px, vs FP64 {u64:.3g} px (FP32 path vs FP64 {uown:.3g}); "
               f"gradients vs FP32 {g32:.3g}, vs FP64 {g64:.3g} (FP32 path vs FP64 {gown:.3g}; bar 1e-6)",
               flush=True)
-        assert u32 <= 1e-4 and u64 <= 1e-4, (u32, u64)
+        spacing = float(np.spacing(np.float32(poi.max())))
+        assert u64 <= 1e-4, u64
+        assert u32 <= max(1e-4, 1.5 * spacing), (u32, spacing)
         assert g64 <= max(1e-6, 1.5 * gown), (g64, gown)
         assert g32 <= max(1e-6, 2.0 * gown), (g32, gown)

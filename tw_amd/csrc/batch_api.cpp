@@ -62,6 +62,9 @@ struct twdic_batch {
     int64_t nm[4] = {};
     int2* d_wg[2] = {};               // IC-GN workgroup items (pair, first POI): step table fits, does not
     int64_t nwg[2] = {};
+    // persistent form: grid sizes from each kernel's own occupancy, computed
+    // when the pairs are set (never inside a stream capture)
+    int pblocks_state = 0, pblocks_icgn = 0, pblocks_fftcc[3][5] = {};
 };
 
 namespace {
@@ -88,11 +91,8 @@ int fclass(int N) { return N == 64 ? 0 : N == 32 ? 1 : 2; }
 // single-context kernels launch_fftcc and launch_icgn on four streams).  The
 // per-POI reference state is built per pair (launch_ref_state) in both.
 #ifdef TWDIC_BATCH_EXEC
-#define TB_EXEC_GATE(b) (void)0
 constexpr bool kPersistent = true;
 #else
-#define TB_EXEC_GATE(b) \
-    return bfail((b), TWDIC_EINVAL, "the persistent batched form is compiled only with TWDIC_BATCH_EXEC (it stalled on its first run)")
 constexpr bool kPersistent = false;
 #endif
 
@@ -188,7 +188,7 @@ twdic_status build_state(twdic_batch* b)
     }
     TB_HIP(b, launch_fill_zero(b->d_counters + CNT_STATE, 1, b->stream));
     TB_HIP(b, launch_batch_ref_state(b->d_desc, b->d_items, (int)b->total, b->d_counters + CNT_STATE,
-                                     persistent_blocks_icgn(b->device), b->stream));
+                                     b->pblocks_state, b->stream));
     b->state_valid = true;
     return TWDIC_OK;
 }
@@ -204,12 +204,12 @@ twdic_status enqueue_frame(twdic_batch* b, bool fftcc, bool icgn)
             const int N = c == 2 ? 0 : Ns[c];
             TB_HIP(b, launch_batch_fftcc(b->d_desc, b->d_fitems[c], (int)b->nf[c], b->d_counters + CNT_F64 + c, N,
                                          b->tune.fftcc_variant,
-                                         persistent_blocks_fftcc(b->device, N, b->tune.fftcc_variant), b->stream));
+                                         b->pblocks_fftcc[c][b->tune.fftcc_variant], b->stream));
         }
     }
     if (icgn)
         TB_HIP(b, launch_batch_icgn(b->d_desc, b->d_items, (int)b->total, b->d_counters + CNT_ICGN,
-                                    std::max(1, b->tune.icgn_chunk), persistent_blocks_icgn(b->device), b->stream));
+                                    std::max(1, b->tune.icgn_chunk), b->pblocks_icgn, b->stream));
     return TWDIC_OK;
 }
 
@@ -407,6 +407,24 @@ twdic_status set_pairs_impl(twdic_batch* b, const twdic_pair* pairs, int npairs)
         if (!b->nf[c]) continue;
         TB_HIP(b, alloc((void**)&b->d_fitems[c], sizeof(int2) * fitems[c].size()));
         TB_HIP(b, hipMemcpy(b->d_fitems[c], fitems[c].data(), sizeof(int2) * fitems[c].size(), hipMemcpyHostToDevice));
+    }
+    // every work item names a listed pair and one of its POIs (the device
+    // loops trust the lists: a stray item would index past a pair's buffers)
+    auto bad_item = [&](const std::vector<int2>& v, int step) {
+        for (const int2& it : v)
+            if (it.x < 0 || it.x >= npairs || it.y < 0 || it.y >= b->pairs[it.x].n || it.y % step) return true;
+        return false;
+    };
+    bool bad = bad_item(items, 1) || bad_item(wg[0], 4) || bad_item(wg[1], 4);
+    for (int f = 0; f < 4 && !bad; ++f) bad = bad_item(mitems[f], 1);
+    for (int c = 0; c < 3 && !bad; ++c) bad = bad_item(fitems[c], 1);
+    if (bad) return bfail(b, TWDIC_EINVAL, "internal: a work item names no POI of its pair");
+    if (kPersistent) {
+        const int Ns[3] = {64, 32, 0};
+        b->pblocks_state = persistent_blocks_state(b->device);
+        b->pblocks_icgn = persistent_blocks_icgn(b->device);
+        for (int c = 0; c < 3; ++c)
+            for (int v = 2; v <= 4; ++v) b->pblocks_fftcc[c][v] = persistent_blocks_fftcc(b->device, Ns[c], v);
     }
     b->state_valid = false;
     b->graph_dirty = true;

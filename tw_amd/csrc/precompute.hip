@@ -348,6 +348,14 @@ __global__ __launch_bounds__(256) void k_batch_ref_state(const PairDesc* __restr
     }
 }
 
+int persistent_blocks_state(int device)
+{
+    int cus = 0, per = 0;
+    (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device);
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_batch_ref_state, 256, 0) != hipSuccess || per < 1) per = 1;
+    return (cus > 0 ? cus : 256) * per;
+}
+
 hipError_t launch_batch_ref_state(const PairDesc* pairs, const int2* items, int total, int* counter, int blocks,
                                   hipStream_t st)
 {

@@ -147,6 +147,7 @@ hipError_t launch_track_accumulate_uv(int n, const float* u, const float* v, flo
                                       hipStream_t st);
 // occupancy-derived persistent grid sizes
 int persistent_blocks_icgn(int device);
+int persistent_blocks_state(int device);
 int persistent_blocks_fftcc(int device, int N, int variant);
 
 }  // namespace twdic
