@@ -162,6 +162,20 @@ twdic_status twdic_icgn(twdic_ctx* ctx, const int32_t* d_poi_yx, int n,
                         float* d_u, float* d_v, float* d_p6, int32_t* d_iters,
                         uint8_t* d_flags, float* d_dpnorm);
 
+/* twdic_icgn without allocation: no counterpart in the reference.  For
+ * callers that cannot allocate device memory: code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc.  Same arguments and
+ * results (bit for bit) as twdic_icgn without d_dpnorm; it calls no
+ * hipMalloc/hipFree.  The per-POI reference state lives in one fixed device
+ * array per GPU (24 MiB, held until the process ends) instead of a context
+ * buffer, so it is rebuilt on every call and nothing is cached between calls;
+ * at most 65536 POIs per call (TWDIC_EINVAL beyond).  Every call on a GPU
+ * shares that array: calls that overlap on one GPU (other contexts, other
+ * streams) race and must be ordered by the caller; calls on one context are
+ * ordered by its stream. */
+twdic_status twdic_icgn_noalloc(twdic_ctx* ctx, const int32_t* d_poi_yx, int n, float* d_u, float* d_v,
+                                float* d_p6, int32_t* d_iters, uint8_t* d_flags);
+
 /* Marks the cached per-POI reference state stale (rebuilt by the next
  * twdic_icgn); the reference rebuilds it only on a reference update
  * (TW_paDIC_cuICGN2D.cu:883-903), so it has no counterpart call. */

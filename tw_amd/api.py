@@ -101,6 +101,7 @@ _SIGS = {
     "twdic_poi_grid": ([_VP, _VP], _I),
     "twdic_fftcc": ([_VP, _VP, _I, _VP, _VP, _VP, _VP], _I),
     "twdic_icgn": ([_VP, _VP, _I, _VP, _VP, _VP, _VP, _VP, _VP], _I),
+    "twdic_icgn_noalloc": ([_VP, _VP, _I, _VP, _VP, _VP, _VP, _VP], _I),
     "twdic_frame": ([_VP, _VP, _I, _VP, _VP, _VP, _VP, _VP, _VP], _I),
     "twdic_frame_capture": ([_VP, _VP, _I, _VP, _VP, _VP, _VP, _VP, _VP], _I),
     "twdic_frame_replay": ([_VP], _I),
@@ -358,6 +359,15 @@ class Context:
         g = lambda a: a.ptr if a is not None else None  # noqa: E731
         self._check(lib().twdic_icgn(self.h, poi.ptr, n, u.ptr, v.ptr, g(p6), g(iters), g(flags), g(dpnorm)),
                     "icgn")
+
+    def icgn_noalloc(self, poi: DeviceArray, n: int, u: DeviceArray, v: DeviceArray, p6=None, iters=None,
+                     flags=None):
+        """twdic_icgn_noalloc: IC-GN with the per-POI state in the library's
+        fixed per-GPU scratch (no device allocation; at most 65536 POIs)."""
+        _fits(n, poi=poi, u=u, v=v, p6=p6, iters=iters, flags=flags)
+        g = lambda a: a.ptr if a is not None else None  # noqa: E731
+        self._check(lib().twdic_icgn_noalloc(self.h, poi.ptr, n, u.ptr, v.ptr, g(p6), g(iters), g(flags)),
+                    "icgn_noalloc")
 
     def frame(self, poi, n, u, v, zncc, p6=None, iters=None, flags=None):
         _fits(n, poi=poi, u=u, v=v, zncc=zncc, p6=p6, iters=iters, flags=flags)

@@ -8,6 +8,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <initializer_list>
+#include <mutex>
 #include <new>
 #include <string>
 
@@ -468,6 +469,82 @@ twdic_status twdic_icgn(twdic_ctx* c, const int32_t* d_poi, int n, float* d_u, f
     IcgnParams p;
     twdic_status st = icgn_prepare(c, d_poi, n, d_u, d_v, d_p6, d_iters, d_flags, d_dpnorm, p);
     if (st != TWDIC_OK) return st;
+    TW_HIP(c, launch_icgn(p, c->cfg.interp, c->icgn_variant, c->stream));
+    return TWDIC_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// IC-GN without allocation.  The per-POI reference state (Cholesky factor,
+// steepest-descent sums: kRefStateStride doubles per POI), which twdic_icgn
+// keeps in a context buffer it grows with hipMalloc, lives here in one
+// file-scope device array per GPU.  In host code `scratch` names a host-side
+// stand-in, not device memory: its device address comes from
+// hipGetSymbolAddress, once per device, and only that address reaches a
+// kernel.
+// ---------------------------------------------------------------------------
+namespace {
+
+constexpr int kNoAllocMaxPois = 65536;       // 65536 x 384 B = 24 MiB per GPU
+static __device__ __align__(256) double scratch[(size_t)kNoAllocMaxPois * kRefStateStride];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[kMaxDevices] = {};
+
+// The device address of `scratch` on the context's device (looked up on the
+// first call there, with that device current).
+twdic_status scratch_for(twdic_ctx* c, double** out)
+{
+    if (c->device < 0 || c->device >= kMaxDevices) return fail(c, TWDIC_EINVAL, "device index out of range");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[c->device]) {
+        TW_HIP(c, hipSetDevice(c->device));
+        void* p = nullptr;
+        TW_HIP(c, hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (!p) return fail(c, TWDIC_EHIP, "hipGetSymbolAddress returned no address for the scratch array");
+        g_scratch_ptr[c->device] = static_cast<double*>(p);
+    }
+    *out = g_scratch_ptr[c->device];
+    return TWDIC_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+twdic_status twdic_icgn_noalloc(twdic_ctx* c, const int32_t* d_poi, int n, float* d_u, float* d_v, float* d_p6,
+                                int32_t* d_iters, uint8_t* d_flags)
+{
+    if (!c || n < 0 || (n > 0 && (!d_poi || !d_u || !d_v)))
+        return fail(c, TWDIC_EINVAL, "twdic_icgn_noalloc: bad arguments");
+    if (n > kNoAllocMaxPois)
+        return fail(c, TWDIC_EINVAL, "twdic_icgn_noalloc: at most 65536 POIs per call (24 MiB of per-GPU scratch)");
+    if (!c->ref_loaded || !c->tar_loaded) return fail(c, TWDIC_EINVAL, "twdic_icgn_noalloc: images not set");
+    if (n == 0) return TWDIC_OK;
+    const size_t m = (size_t)n;
+    TW_SPANS(c, "twdic_icgn_noalloc", {d_poi, 8 * m, "d_poi"}, {d_u, 4 * m, "d_u"}, {d_v, 4 * m, "d_v"},
+             {d_p6, 24 * m, "d_p6"}, {d_iters, 4 * m, "d_iters"}, {d_flags, m, "d_flags"});
+    double* state = nullptr;
+    twdic_status st = scratch_for(c, &state);
+    if (st != TWDIC_OK) return st;
+    st = ensure_ref_grads(c);
+    if (st != TWDIC_OK) return st;
+    const twdic_config& g = c->cfg;
+    if (g.interp == TWDIC_INTERP_LUT && !c->tar_pre_valid) {
+        TW_HIP(c, launch_bicubic_lut(c->d_tar, c->dstride, g.roi_x, g.roi_y, g.roi_w, g.roi_h, c->d_lut, c->stream));
+        c->tar_pre_valid = true;
+    } else if (g.interp == TWDIC_INTERP_BSPLINE && !c->tar_pre_valid) {
+        TW_HIP(c, launch_bspline_lut(c->d_tar, c->dstride, g.roi_x, g.roi_y, g.roi_w, g.roi_h, c->d_lut, c->stream));
+        c->tar_pre_valid = true;
+    }
+    // the scratch holds whatever the last call on this GPU left: the state is
+    // rebuilt for every POI of this call (each POI's wavefront writes every
+    // field IC-GN reads), then the iterations read it, in stream order
+    IcgnParams p = icgn_params(c, d_poi, n, d_u, d_v, d_p6, d_iters, d_flags, nullptr);
+    p.refstate = state;
+    TW_HIP(c, launch_ref_state(p, c->stream));
     TW_HIP(c, launch_icgn(p, c->cfg.interp, c->icgn_variant, c->stream));
     return TWDIC_OK;
 }
