@@ -34,8 +34,11 @@
 //     N <= 64, with the same exact re-evaluation.
 #include "twdic_internal.h"
 
+#include <algorithm>
 #include <cfloat>
 #include <climits>
+#include <cmath>
+#include <mutex>
 
 namespace twdic {
 
@@ -1121,6 +1124,412 @@ __global__ __launch_bounds__(64) void k_batch_fftcc_mma(const PairDesc* __restri
     fftcc_mma_poi<NT, N64, TILE>(sm, p, __builtin_amdgcn_readfirstlane(it.y));
 }
 
+// ===========================================================================
+// Matrix-core DFT engine, N = 64: the correlation surface from fp16 2-D DFTs
+// on v_mfma_f32_32x32x16_f16, then the exact integer correlation at every
+// shift the surface leaves within its proven error of the maximum.
+//
+//   r = R - a, t = T - b with a, b the rounded window means: integers in
+//   [-255, 255], exact in fp16; the correlation of r and t differs from I[k]
+//   = sum R T(k) by a constant, so it has the same argmax.
+//   Forward: R^ = F r F, T^ = F t F (F[j][y] = w^(j y), w = exp(-2 pi i / 64),
+//   rounded to fp16), two passes of fp16 MFMAs with fp32 accumulation, the
+//   intermediate rounded to fp16.  r and t are real, so only the columns k =
+//   0 .. 32 are needed (column 64 - k is the conjugate of column k, rows
+//   negated): the first pass computes k = 0 .. 31 with column 0 carrying Y[.][0]
+//   + i Y[.][32] (its table column is 1 + i (-1)^x, exact), and after the
+//   second pass W[j] = R^[j][0] + i R^[j][32] is unpacked with its partner
+//   row -j: R^[j][0] = (W[j] + conj W[-j]) / 2, R^[j][32] = (W[j] - conj
+//   W[-j]) / 2i.  P = conj(R^) T^ s in fp32 (s a power of two putting |P|
+//   below 2^14), rounded to fp16, column 0 repacked as P[.][0] + i P[.][32].
+//   Inverse: Q = P^T F* for k = 0 .. 31 (scaled by 2^-8, rows k >= 1 doubled
+//   for their conjugates), row 0 giving Q[0][a] + i Q[32][a]; then C^T =
+//   Re(F* Q) + (-1)^b Q[32][a]: 144 MFMAs per POI against the integer
+//   engine's 512 of the same length.
+//   Layouts: a 32 x 32 x 16 accumulator holds column n in lane n (+32 h)
+//   and rows 8 (r / 4) + 4 h + r % 4 in register r, so registers 8 c' .. 8 c'
+//   + 7 of a tile are a ready B (or, for a transposed product, A) operand
+//   whose 16 K values are those rows in that order; the other operand, the
+//   fp16 DFT matrix, is stored with its K index in the same order (table 1),
+//   so no pass transposes through LDS.
+//   Error: |C^ - C| <= kDftBound ||r|| ||t|| / N^2 everywhere.  The bound
+//   follows from ||fp16(F) - F||_2 = 1.29e-3, the largest entry error of
+//   fp16(F) (1.73e-4), the fp16 and fp32 unit roundoffs, the worst-case error
+//   of an fp32 sum of n terms (gamma_{n-1} sum |terms|) for each pass and
+//   Cauchy-Schwarz; completing the half spectrum by conjugation turns the
+//   computation into the full-spectrum one with one set of errors.  It is
+//   15.1 with fp16 sub-normals kept and 21.7 if they were flushed to zero;
+//   kDftBound covers both, about sixteen times the largest error of a
+//   matrix-level model of this arithmetic on stress windows.  Every shift
+//   whose approximate value is within twice that (x 1.25) of the approximate
+//   maximum is evaluated exactly in integers, in increasing index order,
+//   keeping the first strict maximum: the exact argmax with the CPU path's
+//   lowest-index tie-break.  (Speckle windows leave about two such shifts; a
+//   window of pure noise a handful; the count is not capped.)
+//   A workgroup of eight wavefronts, one POI each, shares the fp16 tables in
+//   LDS; the window bytes stay in LDS for the exact stage.
+// ===========================================================================
+typedef _Float16 h8 __attribute__((ext_vector_type(8)));
+typedef _Float16 h2v __attribute__((ext_vector_type(2)));
+typedef float f16x __attribute__((ext_vector_type(16)));
+
+constexpr double kDftBound = 22.0;        // N^2 |C^ - C| / (||r|| ||t||), see above
+constexpr int kExR = 80, kExT = 144;      // LDS pitch of the R rows (64 B) and the doubled T rows (128 B)
+constexpr int kDftWaves = 8;              // POIs in flight per workgroup (one per wavefront)
+
+// F in fp16 as MFMA operands, [table][part][tile][K chunk][lane], lane = 32 h + m,
+// parts re, im, -im:
+//   table 0: F[16 c + 8 h + q][32 t + m]                 (B operand, K in natural order);
+//            in tile 0 the imaginary part of column 0 holds (-1)^x instead of 0
+//   table 1: F[32 t + m][16 c + 8 (q / 4) + 4 h + q % 4]   (A operand, K in accumulator row order)
+__device__ h8 g_dft_tab[2][3][2][4][64];
+
+// One workgroup: the tables (32 KB, read by all its wavefronts) and, per
+// wavefront, its POI's R rows and doubled T rows as bytes and the second
+// pass's spectrum column 0 (for the j <-> -j unpacking).
+struct DftWgSmem {
+    h8 ta[2][4][64];
+    h8 tb[3][2][4][64];
+    uint8_t ex[kDftWaves][64 * kExR + 64 * kExT];
+    float4 col0[kDftWaves][64];
+    uint32_t xcol[kDftWaves][64];
+};
+
+__device__ __forceinline__ f16x mfma16(h8 a, h8 b, f16x c)
+{
+    return __builtin_amdgcn_mfma_f32_32x32x16_f16(a, b, c, 0, 0, 0);
+}
+
+// accumulator registers 8 half .. 8 half + 7, times scale, rounded to fp16
+__device__ __forceinline__ h8 to_h8(const f16x& acc, int half, float scale)
+{
+    h8 r;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) r[q] = (_Float16)(acc[8 * half + q] * scale);
+    return r;
+}
+
+// LDS written by lanes of this wavefront and then read by other lanes of it:
+// its LDS accesses complete in issue order; keep the compiler from moving
+// them across this point.
+__device__ __forceinline__ void wave_lds_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Eight window bytes as the fp16 values byte - off: the byte v becomes the
+// half 1024 + v (bits 0x6400 | v), minus off = 1024 + mean, exactly.  The
+// byte shuffle is one v_perm_b32 per two values (selector bytes 0-3 take the
+// bytes of the second source, 4-7 those of the first: 0x64).
+__device__ __forceinline__ h8 centred_h8(uint2 w, h2v off)
+{
+    const uint32_t q[4] = {__builtin_amdgcn_perm(0x64646464u, w.x, 0x04010400u),
+                           __builtin_amdgcn_perm(0x64646464u, w.x, 0x04030402u),
+                           __builtin_amdgcn_perm(0x64646464u, w.y, 0x04010400u),
+                           __builtin_amdgcn_perm(0x64646464u, w.y, 0x04030402u)};
+    h8 r;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const h2v v = __builtin_bit_cast(h2v, q[k]) - off;
+        r[2 * k] = v[0];
+        r[2 * k + 1] = v[1];
+    }
+    return r;
+}
+
+__device__ void fftcc_dft_mfma_poi(DftWgSmem& sm, int wave, int lane, const FftccParams& p, int poi)
+{
+    constexpr int N = 64, NN = 4096;
+    const int m = lane & 31, h = lane >> 5;
+    const int py = p.poi[2 * poi], px = p.poi[2 * poi + 1];
+    if (py - N / 2 < 0 || px - N / 2 < 0 || py + N / 2 > p.img_h || px + N / 2 > p.img_w) {
+        if (lane == 0 && p.status) p.status[poi] = TWDIC_FFTCC_OUT;
+        return;
+    }
+    uint8_t* ex = sm.ex[wave];
+    const uint8_t* erows = ex;                 // R row y at y kExR
+    const uint8_t* etrows = ex + 64 * kExR;    // T row y at y kExT, twice
+    float4* col0 = sm.col0[wave];
+    uint32_t* xcol = sm.xcol[wave];
+    // -- window rows (lane = row) to LDS, exact integer sums -------------------
+    int sr = 0, st = 0, srr = 0, stt = 0;
+    {
+        const size_t o = (size_t)(py - N / 2 + lane) * p.stride + (px - N / 2);
+        uint32_t rw[16], tw[16];
+        __builtin_memcpy(rw, p.ref + o, 64);
+        __builtin_memcpy(tw, p.tar + o, 64);
+#pragma unroll
+        for (int k = 0; k < 16; ++k) {
+            sr = (int)__builtin_amdgcn_udot4(rw[k], 0x01010101u, (uint32_t)sr, false);
+            st = (int)__builtin_amdgcn_udot4(tw[k], 0x01010101u, (uint32_t)st, false);
+            srr = (int)__builtin_amdgcn_udot4(rw[k], rw[k], (uint32_t)srr, false);
+            stt = (int)__builtin_amdgcn_udot4(tw[k], tw[k], (uint32_t)stt, false);
+        }
+        wave_lds_sync();                       // the previous POI's reads of this region are done
+        uint4* er = reinterpret_cast<uint4*>(ex + lane * kExR);
+        uint4* et = reinterpret_cast<uint4*>(ex + 64 * kExR + lane * kExT);
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const uint4 a = make_uint4(rw[4 * k], rw[4 * k + 1], rw[4 * k + 2], rw[4 * k + 3]);
+            const uint4 b = make_uint4(tw[4 * k], tw[4 * k + 1], tw[4 * k + 2], tw[4 * k + 3]);
+            er[k] = a;
+            et[k] = b;
+            et[4 + k] = b;
+        }
+    }
+    sr = wave_sum_i(sr); st = wave_sum_i(st); srr = wave_sum_i(srr); stt = wave_sum_i(stt);
+    const long long Sr = sr, St = st, Srr = srr, Stt = stt;
+    if (Srr * NN == Sr * Sr || Stt * NN == St * St) {          // flat window (:271-275)
+        if (lane == 0 && p.status) p.status[poi] = TWDIC_FFTCC_FLAT;
+        return;
+    }
+    const int a = (sr + NN / 2) >> 12, b = (st + NN / 2) >> 12;    // rounded means
+    const h2v oa = {(_Float16)(1024 + a), (_Float16)(1024 + a)};
+    const h2v ob = {(_Float16)(1024 + b), (_Float16)(1024 + b)};
+    const double nr2 = (double)(Srr - 2LL * a * Sr + (long long)a * a * NN);     // ||r||^2, exact
+    const double nt2 = (double)(Stt - 2LL * b * St + (long long)b * b * NN);
+    const double rho = sqrt(nr2 * nt2);
+    int ex2;
+    (void)frexp(4096.0 * rho, &ex2);
+    const float s = ldexpf(1.0f, 14 - ex2);                                      // s 4096 rho in [2^13, 2^14)
+    const float tol = (float)(2.0 * 1.25 * kDftBound * rho * (double)s / 256.0); // in C^ units (s 2^-8 N^2 C)
+    wave_lds_sync();
+
+    // -- first pass, columns k = 0 .. 31 (column 0 = Y[.][0] + i Y[.][32]) -------
+    h8 Yrr[4], Yri[4], Ytr[4], Yti[4];         // B operands, K chunk c = rows 16 c .. 16 c + 15 (accumulator order)
+#pragma unroll
+    for (int Mt = 0; Mt < 2; ++Mt) {
+        f16x yrr = {}, yri = {}, ytr = {}, yti = {};
+        const int y = 32 * Mt + m;
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            const h8 Fr = sm.ta[0][c][lane], Fi = sm.ta[1][c][lane];
+            const h8 Ar = centred_h8(*reinterpret_cast<const uint2*>(erows + y * kExR + 16 * c + 8 * h), oa);
+            const h8 At = centred_h8(*reinterpret_cast<const uint2*>(etrows + y * kExT + 16 * c + 8 * h), ob);
+            yrr = mfma16(Ar, Fr, yrr);
+            yri = mfma16(Ar, Fi, yri);
+            ytr = mfma16(At, Fr, ytr);
+            yti = mfma16(At, Fi, yti);
+        }
+#pragma unroll
+        for (int hf = 0; hf < 2; ++hf) {
+            Yrr[2 * Mt + hf] = to_h8(yrr, hf, 1.f);
+            Yri[2 * Mt + hf] = to_h8(yri, hf, 1.f);
+            Ytr[2 * Mt + hf] = to_h8(ytr, hf, 1.f);
+            Yti[2 * Mt + hf] = to_h8(yti, hf, 1.f);
+        }
+        __builtin_amdgcn_sched_barrier(0);
+    }
+    // -- second pass: R^, T^ for k = 0 .. 31 and P = conj(R^) T^ s ----------------
+    h8 Pre[4], Pim[4];                         // A operands of the first inverse pass [j chunk]
+#pragma unroll
+    for (int Mj = 0; Mj < 2; ++Mj) {
+        f16x Rr = {}, Ri = {}, Tr = {}, Ti = {};
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            const h8 Fr = sm.tb[0][Mj][c][lane], Fi = sm.tb[1][Mj][c][lane], nFi = sm.tb[2][Mj][c][lane];
+            Rr = mfma16(Fr, Yrr[c], Rr);
+            Ri = mfma16(Fr, Yri[c], Ri);
+            Tr = mfma16(Fr, Ytr[c], Tr);
+            Ti = mfma16(Fr, Yti[c], Ti);
+            Rr = mfma16(nFi, Yri[c], Rr);
+            Ri = mfma16(Fi, Yrr[c], Ri);
+            Tr = mfma16(nFi, Yti[c], Tr);
+            Ti = mfma16(Fi, Ytr[c], Ti);
+        }
+        if (m == 0) {                          // column 0: W_R = R^[j][0] + i R^[j][32], W_T likewise
+#pragma unroll
+            for (int r = 0; r < 16; ++r)
+                col0[32 * Mj + 8 * (r >> 2) + 4 * h + (r & 3)] = make_float4(Rr[r], Ri[r], Tr[r], Ti[r]);
+        }
+        f16x pr, pi;
+#pragma unroll
+        for (int i = 0; i < 16; ++i) {
+            pr[i] = fmaf(Rr[i], Tr[i], Ri[i] * Ti[i]);
+            pi[i] = fmaf(Rr[i], Ti[i], -(Ri[i] * Tr[i]));
+        }
+#pragma unroll
+        for (int hf = 0; hf < 2; ++hf) {
+            Pre[2 * Mj + hf] = to_h8(pr, hf, s);
+            Pim[2 * Mj + hf] = to_h8(pi, hf, s);
+        }
+        __builtin_amdgcn_sched_barrier(0);
+    }
+    // column 0 of P becomes P[j][0] + i P[j][32]: unpack R^[j][0] = (W[j] +
+    // conj W[-j]) / 2, R^[j][32] = (W[j] - conj W[-j]) / 2i, likewise T^; lane
+    // j does row j (the results for j and -j are exact conjugates), lanes m = 0
+    // collect theirs
+    wave_lds_sync();
+    {
+        const float4 w = col0[lane], wm = col0[(64 - lane) & 63];
+        const float ar = 0.5f * (w.x + wm.x), ai = 0.5f * (w.y - wm.y);    // R^[j][0]
+        const float br = 0.5f * (w.y + wm.y), bi = 0.5f * (wm.x - w.x);    // R^[j][32]
+        const float cr = 0.5f * (w.z + wm.z), ci = 0.5f * (w.w - wm.w);    // T^[j][0]
+        const float dr = 0.5f * (w.w + wm.w), di = 0.5f * (wm.z - w.z);    // T^[j][32]
+        const float p0r = fmaf(ar, cr, ai * ci), p0i = fmaf(ar, ci, -(ai * cr));   // conj(R^0) T^0
+        const float p1r = fmaf(br, dr, bi * di), p1i = fmaf(br, di, -(bi * dr));   // conj(R^32) T^32
+        const h2v x = {(_Float16)((p0r - p1i) * s), (_Float16)((p0i + p1r) * s)};
+        xcol[lane] = __builtin_bit_cast(uint32_t, x);
+    }
+    wave_lds_sync();
+    if (m == 0) {
+#pragma unroll
+        for (int Mj = 0; Mj < 2; ++Mj)
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                const h2v x = __builtin_bit_cast(h2v, xcol[32 * Mj + 8 * (r >> 2) + 4 * h + (r & 3)]);
+                Pre[2 * Mj + (r >> 3)][r & 7] = x[0];
+                Pim[2 * Mj + (r >> 3)][r & 7] = x[1];
+            }
+    }
+    // -- first inverse pass, rows k = 0 .. 31: Q[k][a] = sum_j P[j][k] conj(F[j][a])
+    //    (row 0 comes out as Q[0][a] + i Q[32][a], both real) -----------------------
+    h8 Qr[2][2], Qi[2][2];                     // B operands of the last pass [k chunk][a tile]
+    float q32[2];                              // Q[32][a] of this lane's column a, per a tile
+    {
+        f16x qr[2] = {}, qi[2] = {};
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+#pragma unroll
+            for (int Nc = 0; Nc < 2; ++Nc) {
+                const h8 Fr = sm.tb[0][Nc][c][lane], Fi = sm.tb[1][Nc][c][lane], nFi = sm.tb[2][Nc][c][lane];
+                qr[Nc] = mfma16(Pre[c], Fr, qr[Nc]);
+                qi[Nc] = mfma16(Pim[c], Fr, qi[Nc]);
+                qr[Nc] = mfma16(Pim[c], Fi, qr[Nc]);
+                qi[Nc] = mfma16(Pre[c], nFi, qi[Nc]);
+            }
+        }
+#pragma unroll
+        for (int Nc = 0; Nc < 2; ++Nc) {
+            // row k = 0 is register 0 of the lanes h = 0
+            q32[Nc] = __shfl(qi[Nc][0], m) * (1.f / 256.f);
+            // the last pass sums k = 0 .. 31 with weight 1 for k = 0 and 2 for
+            // the others (the conjugate rows 33 .. 63)
+            const float w0 = h == 0 ? 1.f / 256.f : 2.f / 256.f;
+#pragma unroll
+            for (int hf = 0; hf < 2; ++hf) {
+                h8 vr, vi;
+#pragma unroll
+                for (int q = 0; q < 8; ++q) {
+                    const float sc = (hf == 0 && q == 0) ? w0 : 2.f / 256.f;
+                    vr[q] = (_Float16)(qr[Nc][8 * hf + q] * sc);
+                    vi[q] = (_Float16)(qi[Nc][8 * hf + q] * sc);
+                }
+                Qr[hf][Nc] = vr;
+                Qi[hf][Nc] = vi;
+            }
+        }
+        __builtin_amdgcn_sched_barrier(0);
+    }
+    // -- last pass: C^T = Re(F* Q) + (-1)^b Q[32][a], lane -> a, register -> b ------
+    f16x Cd[2][2] = {};                        // [b tile][a tile]
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+#pragma unroll
+        for (int Md = 0; Md < 2; ++Md) {
+            const h8 Fr = sm.tb[0][Md][c][lane], Fi = sm.tb[1][Md][c][lane];
+#pragma unroll
+            for (int Nd = 0; Nd < 2; ++Nd) {
+                Cd[Md][Nd] = mfma16(Fr, Qr[c][Nd], Cd[Md][Nd]);
+                Cd[Md][Nd] = mfma16(Fi, Qi[c][Nd], Cd[Md][Nd]);
+            }
+        }
+    }
+#pragma unroll
+    for (int Md = 0; Md < 2; ++Md)
+#pragma unroll
+        for (int Nd = 0; Nd < 2; ++Nd)
+#pragma unroll
+            for (int r = 0; r < 16; ++r) Cd[Md][Nd][r] += (r & 1) ? -q32[Nd] : q32[Nd];
+
+    // -- candidates: every shift within tol of the approximate maximum -----------
+    float mx = -FLT_MAX;
+#pragma unroll
+    for (int Md = 0; Md < 2; ++Md)
+#pragma unroll
+        for (int Nd = 0; Nd < 2; ++Nd)
+#pragma unroll
+            for (int r = 0; r < 16; ++r) mx = fmaxf(mx, Cd[Md][Nd][r]);
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) mx = fmaxf(mx, __shfl_xor(mx, off));
+    const float thr = mx - tol;
+    // bit 16 (2 Nd + Md) + r: this lane's shifts in increasing row-major index a N + b
+    uint32_t mlo = 0, mhi = 0;
+#pragma unroll
+    for (int Md = 0; Md < 2; ++Md)
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+            mlo |= (uint32_t)(Cd[Md][0][r] >= thr) << (16 * Md + r);
+            mhi |= (uint32_t)(Cd[Md][1][r] >= thr) << (16 * Md + r);
+        }
+    uint64_t mask = ((uint64_t)mhi << 32) | mlo;
+
+    // -- exact integer correlation at the candidates, in increasing index order ---
+    const uint32_t* rrow = reinterpret_cast<const uint32_t*>(erows + lane * kExR);
+    int bestI = -1, peak = -1;
+    for (;;) {
+        int mine = INT_MAX;
+        if (mask) {
+            const int bit = __builtin_ctzll(mask);
+            const int Nd = bit >> 5, Md = (bit >> 4) & 1, r = bit & 15;
+            mine = (32 * Nd + m) * N + 32 * Md + 8 * (r >> 2) + 4 * h + (r & 3);
+        }
+        const int k = wave_min_i(mine);
+        if (k == INT_MAX) break;
+        const int kv = k >> 6, ku = k & 63, d = ku >> 2, sh = ku & 3;
+        const uint32_t* trow = reinterpret_cast<const uint32_t*>(etrows + ((lane + kv) & 63) * kExT);
+        uint32_t acc = 0;
+        uint32_t w0 = trow[d];
+#pragma unroll
+        for (int j = 0; j < 16; ++j) {
+            const uint32_t w1 = trow[d + j + 1];
+            acc = __builtin_amdgcn_udot4(rrow[j], __builtin_amdgcn_alignbyte(w1, w0, sh), acc, false);
+            w0 = w1;
+        }
+        const int I = wave_sum_i((int)acc);
+        if (I > bestI) {
+            bestI = I;
+            peak = k;
+        }
+        if (mine == k) mask &= mask - 1;
+    }
+    if (lane == 0) {
+        const double nr = (double)Srr - (double)Sr * (double)Sr / (double)NN;
+        const double nt = (double)Stt - (double)St * (double)St / (double)NN;
+        int iu, iv;
+        wrap_peak(peak, N, iu, iv);
+        p.u[poi] = (float)iu;
+        p.v[poi] = (float)iv;
+        p.zncc[poi] = (float)(((double)bestI - (double)Sr * (double)St / (double)NN) / sqrt(nr * nt));
+        if (p.status) p.status[poi] = TWDIC_FFTCC_OK;
+    }
+}
+
+// Workgroups of kDftWaves wavefronts share the DFT tables in LDS; workgroup
+// b takes the contiguous POI range [b n / G, (b + 1) n / G) (G workgroups),
+// its wavefronts every kDftWaves-th POI of it.
+__global__ __launch_bounds__(64 * kDftWaves, 1) void k_fftcc_dft_mfma(FftccParams p)
+{
+    __shared__ DftWgSmem sm;
+    {
+        const h8* gb = &g_dft_tab[1][0][0][0][0];
+        h8* la = &sm.ta[0][0][0];
+        h8* lb = &sm.tb[0][0][0][0];
+        for (int i = threadIdx.x; i < 2 * 4 * 64; i += blockDim.x)        // table 0, tile 0, parts re and im
+            la[i] = g_dft_tab[0][i >> 8][0][(i >> 6) & 3][i & 63];
+        for (int i = threadIdx.x; i < 3 * 2 * 4 * 64; i += blockDim.x) lb[i] = gb[i];
+    }
+    __syncthreads();
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int lo = (int)((long long)blockIdx.x * p.n / gridDim.x);
+    const int hi = (int)((long long)(blockIdx.x + 1) * p.n / gridDim.x);
+    for (int poi = lo + wave; poi < hi; poi += kDftWaves) fftcc_dft_mfma_poi(sm, wave, lane, p, poi);
+}
+
+int g_dft_cus[64] = {};                        // compute units per device, set by dft_tables_init
+
 }  // namespace
 
 int mma_form(int N)
@@ -1146,7 +1555,13 @@ hipError_t launch_fftcc(const FftccParams& p, int variant, hipStream_t st)
 {
     if (p.n <= 0) return hipSuccess;
     const int pairs = (p.n + 1) / 2;
-    if (variant == kFftccMma) {
+    if (variant == kFftccDftMfma && p.N == 64) {
+        int dev = 0;
+        (void)hipGetDevice(&dev);
+        const int cus = dev >= 0 && dev < 64 && g_dft_cus[dev] > 0 ? g_dft_cus[dev] : 256;
+        const int blocks = std::max(1, std::min(cus, (p.n + kDftWaves - 1) / kDftWaves));
+        hipLaunchKernelGGL(k_fftcc_dft_mfma, dim3(blocks), dim3(64 * kDftWaves), 0, st, p);
+    } else if (variant == kFftccMma || variant == kFftccDftMfma) {
         if (p.N == 64)
             hipLaunchKernelGGL((k_fftcc_mma<64, true, 32>), dim3(p.n), dim3(64), 0, st, p);
         else if (p.N > 48)
@@ -1203,6 +1618,39 @@ int persistent_blocks_fftcc(int device, int N, int variant)
         e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_batch_fftcc_dft, kDftThreads, 0);
     if (e != hipSuccess || per < 1) per = 1;
     return (cus > 0 ? cus : 256) * per;
+}
+
+hipError_t dft_tables_init(int device)
+{
+    static std::mutex mu;
+    static bool done[64] = {};
+    if (device < 0 || device >= 64) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lk(mu);
+    if (done[device]) return hipSuccess;
+    static _Float16 host[2][3][2][4][64][8];
+    for (int t = 0; t < 2; ++t)
+        for (int tile = 0; tile < 2; ++tile)
+            for (int c = 0; c < 4; ++c)
+                for (int lane = 0; lane < 64; ++lane)
+                    for (int q = 0; q < 8; ++q) {
+                        const int m = lane & 31, h = lane >> 5;
+                        const int row = t == 0 ? 16 * c + 8 * h + q : 32 * tile + m;
+                        const int col = t == 0 ? 32 * tile + m : 16 * c + 8 * (q / 4) + 4 * h + q % 4;
+                        const double ang = -2.0 * M_PI * (double)((row * col) & 63) / 64.0;
+                        host[t][0][tile][c][lane][q] = (_Float16)std::cos(ang);   // direct double -> fp16, nearest-even
+                        host[t][1][tile][c][lane][q] = (_Float16)std::sin(ang);
+                        if (t == 0 && tile == 0 && m == 0)                        // (-1)^x: Y[.][32]
+                            host[t][1][tile][c][lane][q] = (_Float16)(((16 * c + 8 * h + q) & 1) ? -1.0 : 1.0);
+                        host[t][2][tile][c][lane][q] = -host[t][1][tile][c][lane][q];
+                    }
+    int cur = 0;
+    hipError_t e = hipGetDevice(&cur);
+    if (e == hipSuccess) e = hipDeviceGetAttribute(&g_dft_cus[device], hipDeviceAttributeMultiprocessorCount, device);
+    if (e == hipSuccess) e = hipSetDevice(device);
+    if (e == hipSuccess) e = hipMemcpyToSymbol(HIP_SYMBOL(g_dft_tab), host, sizeof(host));
+    if (e == hipSuccess) e = hipSetDevice(cur);
+    if (e == hipSuccess) done[device] = true;
+    return e;
 }
 
 hipError_t launch_fftcc_generic(const FftccParams& p, hipStream_t st)

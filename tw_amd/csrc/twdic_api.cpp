@@ -321,6 +321,7 @@ twdic_status twdic_create(const twdic_config* cfg, int device, twdic_ctx** out)
     if ((e = hipSetDevice(device)) != hipSuccess) bail(e, "hipSetDevice");
     if (st == TWDIC_OK && (e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking)) != hipSuccess) bail(e, "hipStreamCreate");
     if (st == TWDIC_OK && (e = hipStreamCreateWithFlags(&c->stream2, hipStreamNonBlocking)) != hipSuccess) bail(e, "hipStreamCreate");
+    if (st == TWDIC_OK && (e = dft_tables_init(device)) != hipSuccess) bail(e, "dft_tables_init");
     const size_t img = (size_t)c->dstride * cfg->img_h;
     const size_t roi = (size_t)cfg->roi_w * cfg->roi_h;
     struct { void** p; size_t b; } allocs[] = {

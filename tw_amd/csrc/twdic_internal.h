@@ -114,6 +114,11 @@ hipError_t launch_poi_grid(int32_t* poi, int start_x, int start_y, int nx, int n
 // N <= 64); 2, 3, 4: the FFT register engine's occupancy target (N = 32, 64,
 // else the direct-DFT engine)
 constexpr int kFftccMma = 8;
+// kFftccDftMfma: N = 64 on fp16 matrix-core DFTs with exact re-evaluation
+// of the near-maximal shifts (other N: the integer engine); needs
+// dft_tables_init on the device first
+constexpr int kFftccDftMfma = 9;
+hipError_t dft_tables_init(int device);
 hipError_t launch_fftcc(const FftccParams& p, int variant, hipStream_t st);
 // batched form: the matrix-core kernel over items (pair, POI) of pairs that
 // share a kernel form (mma_form of their window edge N: 0 for N = 64,
