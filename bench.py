@@ -35,6 +35,10 @@ HBM_PEAK_GBS = 8000.0
 FP32_PEAK_TFLOPS = 157.3
 FP32_UNPACKED_PEAK_TFLOPS = 78.6    # without packed FP32 (v_pk_fma_f32), which the kernels here do not use
 INT8_MFMA_PEAK_TOPS = 5000.0
+FP16_MFMA_PEAK_TFLOPS = 2500.0
+# fp16 DFT engine (N = 64): 144 v_mfma_f32_32x32x16_f16 of 32 x 32 x 16 x 2
+# flops per POI, the half-spectrum DFT-by-matrix product with no padding
+DFT_MFMA_FLOPS_PER_POI = 144 * 32 * 32 * 16 * 2
 
 
 def measured_traffic() -> dict:
@@ -477,12 +481,15 @@ def main():
 
     # Roofline of each kernel from its measured launch time; the dominant
     # (longer) one is reported as "roofline".  Work conventions:
-    #   FFT-CC on the matrix cores (default engine): the whole integer
-    #   correlation surface, 2 N^4 int8 operations per POI, against the dense
-    #   int8 MFMA peak; the FFT formulation's F_cc(N) = 3 * 2.5 N^2 log2(N^2)
-    #   + 6 N (N/2 + 1) + 7 N^2 (two forward and one inverse N x N transform,
-    #   spectrum product, moments and peak) is reported beside it, and is the
-    #   measure for the FP32 FFT engine;
+    #   FFT-CC, default engine for N = 64: fp16 DFTs as matrix products,
+    #   DFT_MFMA_FLOPS_PER_POI against the dense fp16 MFMA peak (the exact
+    #   integer re-evaluation of the few near-maximal shifts is not counted);
+    #   the integer engine (other N): the whole integer correlation surface,
+    #   2 N^4 int8 operations per POI, against the dense int8 MFMA peak; the
+    #   FFT formulation's F_cc(N) = 3 * 2.5 N^2 log2(N^2) + 6 N (N/2 + 1) +
+    #   7 N^2 (two forward and one inverse N x N transform, spectrum product,
+    #   moments and peak) is reported beside either, and is the measure for
+    #   the FP32 FFT engine;
     #   IC-GN: F_it(s) = 66 s FP32 operations per iteration over an s-pixel subset.
     s = (2 * case.S + 1) ** 2
     N = case.N
@@ -490,9 +497,12 @@ def main():
     icgn_ms = float(np.mean(t_icgn))
     fft_ms = float(np.mean(t_fft))
     f_cc = 3 * 2.5 * N * N * np.log2(N * N) + 6 * N * (N // 2 + 1) + 7 * N * N
-    mma = os.environ.get("TWDIC_FFTCC_VARIANT", "8") not in ("2", "3", "4")
+    variant = os.environ.get("TWDIC_FFTCC_VARIANT", "9")
+    dft = variant == "9" and N == 64
+    mma = variant not in ("2", "3", "4") and not dft
     kern = {
         "k_fftcc": {"ms": fft_ms, "flops": f_cc * n, "int8_ops": 2.0 * N ** 4 * n if mma else None,
+                    "fp16_flops": float(DFT_MFMA_FLOPS_PER_POI) * n if dft else None,
                     # both images read once, POI in, u/v/ZNCC out
                     "bytes": 2 * W * H + n * (8 + 12)},
         "k_icgn": {"ms": icgn_ms, "flops": 66.0 * s * kbar * n,
@@ -510,6 +520,17 @@ def main():
     D = kern[dom]
 
     def roof(name, d):
+        if d.get("fp16_flops"):
+            tf = d["fp16_flops"] / (d["ms"] * 1e-3) / 1e12
+            return {"bound": "mfma", "pipe": "fp16 MFMA (v_mfma_f32_32x32x16_f16), half-spectrum DFTs as matrix "
+                                             "products, then the exact integer correlation at the near-maximal shifts",
+                    "kernel": name, "achieved": tf, "peak": FP16_MFMA_PEAK_TFLOPS, "unit": "TFLOP/s",
+                    "frac": tf / FP16_MFMA_PEAK_TFLOPS, "traffic": d["traffic"],
+                    "avg_launch_ms": d["ms"], "algorithmic_flops_per_launch": d["fp16_flops"],
+                    "fft_equivalent": {"achieved": d["tflops"], "unit": "TFLOP/s",
+                                       "algorithmic_flops_per_launch": d["flops"]},
+                    "hbm": {"achieved": d["gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                            "frac": d["gbs"] / HBM_PEAK_GBS, "algorithmic_bytes_per_launch": d["bytes"]}}
         if d.get("int8_ops"):
             tops = d["int8_ops"] / (d["ms"] * 1e-3) / 1e12
             shape = "16x16x64" if 32 < N <= 48 else "32x32x32"
@@ -570,7 +591,8 @@ def main():
         "fftcc_pois_per_s": n / (fft_ms * 1e-3),
         "latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99))},
         "kernel_ms": {"fftcc": fft_ms, "icgn": icgn_ms},
-        "fftcc_engine": "int8 matrix cores, exact" if mma else "fp32 FFT with exact re-evaluation",
+        "fftcc_engine": ("fp16 matrix-core DFTs, exact integer re-evaluation of the near-maximal shifts" if dft
+                         else "int8 matrix cores, exact" if mma else "fp32 FFT with exact re-evaluation"),
         "mean_iterations": kbar,
         "flags": np.bincount(gfl, minlength=5).tolist(),
         "subpixel_err_px": {"median": float(np.median(err)), "p99": float(np.percentile(err, 99)),

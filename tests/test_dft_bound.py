@@ -60,7 +60,9 @@ def dft_bound(flush):
          P[:,0] + i P[:,32];
       C  Q = P^T conj(F) for k = 0 .. 31, scaled by 2^-8 and by the weights
          1 (k = 0) / 2 (k >= 1), rounded to fp16;
-      D  C^T = Re(conj(F) Q) over k = 0 .. 31, plus (-1)^b Q[32][a] in fp32.
+      D  C^T = Re(conj(F) Q) over k = 0 .. 31, the imaginary table part of
+         k = 0 (zero in F) set to (-1)^b, so that Q[32][a] (row 0's imaginary
+         part) enters as (-1)^b Q[32][a].
     Completing the half spectrum by conjugation gives exactly the full-spectrum
     computation with one set of errors; the packed column's two halves get
     errors whose squared norms sum to that of the packed column (parallelogram
@@ -87,8 +89,7 @@ def dft_bound(flush):
     inv = pi * (2 * dmax * (1 + dmax)           # table error in C and in D
                 + 2 * g127 * (1 + dmax) ** 2    # C's sums
                 + u * (1 + dmax) ** 2 * (1 + 2 * g127)   # Q to fp16
-                + g63 * (1 + dmax) ** 2 * (1 + u)        # D's sums
-                + 2 * v * (1 + dmax) ** 2)               # the Nyquist term's add
+                + g63 * (1 + dmax) ** 2 * (1 + u))       # D's sums
     inv += 64 * r2 * tiny * 256 * sinv * (1 + dmax)      # Q's sub-normal floor
     return dP + fp32P + roundP + inv
 
@@ -173,12 +174,12 @@ def model_half(r, t):
     Pr, Pi = _f16(pr * s), _f16(pi * s)
     qr = (np.concatenate([Pr.T, Pi.T], 1) @ np.concatenate([Fr, Fi], 0)).astype(np.float32)
     qi = (np.concatenate([Pi.T, Pr.T], 1) @ np.concatenate([Fr, -Fi], 0)).astype(np.float32)
-    q32 = qi[0] / np.float32(256)
     w = np.full((32, 1), 2.0 / 256, np.float32)
     w[0] = 1.0 / 256
     Qr, Qi = _f16(qr * w), _f16(qi * w)
-    ct = (np.concatenate([Fr[:, :32], Fi[:, :32]], 1) @ np.concatenate([Qr, Qi], 0)).astype(np.float32)
-    ct += ((-1.0) ** np.arange(N))[:, None].astype(np.float32) * q32[None, :]
+    FiD = Fi[:, :32].copy()
+    FiD[:, 0] = (-1.0) ** np.arange(N)
+    ct = (np.concatenate([Fr[:, :32], FiD], 1) @ np.concatenate([Qr, Qi], 0)).astype(np.float32)
     return ct.T.astype(np.float64) * 256.0 / float(s) / 4096.0, nr * nt
 
 

@@ -239,7 +239,7 @@ twdic_status run_direct(twdic_batch* b, bool fftcc, bool icgn)
     for (size_t i = 0; i < b->h_desc.size(); ++i) {
         const PairDesc& d = b->h_desc[i];
         hipStream_t s = b->lane[i % L];
-        if (fftcc && d.fftcc.n > 0) TB_HIP(b, launch_fftcc(d.fftcc, kFftccMma, s));
+        if (fftcc && d.fftcc.n > 0) TB_HIP(b, launch_fftcc(d.fftcc, kFftccDftMfma, s));
         if (icgn && d.icgn.n > 0) TB_HIP(b, launch_icgn(d.icgn, TWDIC_INTERP_FUSED, 5, s));
     }
     // join: later work on the batch stream (events, syncs, the next step)
@@ -386,7 +386,7 @@ twdic_status set_pairs_impl(twdic_batch* b, const twdic_pair* pairs, int npairs)
         const twdic_pair& p = b->pairs[i];
         auto& mi = mitems[mma_form(p.fft_n)];
         for (int k = 0; k < p.n; ++k) mi.push_back(make_int2(i, k));
-        auto& w = wg[step_table_fits((2 * p.sx + 1) * (2 * p.sy + 1)) ? 0 : 1];
+        auto& w = wg[step_table_fits(2 * p.sx + 1, 2 * p.sy + 1) ? 0 : 1];
         for (int k = 0; k < p.n; k += 4) w.push_back(make_int2(i, k));
     }
     for (int f = 0; f < 4; ++f) {
@@ -446,6 +446,10 @@ twdic_status twdic_batch_create(int device, twdic_batch** out)
     b->device = device;
     if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) {
         delete b;
+        return TWDIC_EHIP;
+    }
+    if (dft_tables_init(device) != hipSuccess) {
+        twdic_batch_destroy(b);
         return TWDIC_EHIP;
     }
     for (auto& e : b->ev)

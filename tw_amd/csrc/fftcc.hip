@@ -1144,8 +1144,9 @@ __global__ __launch_bounds__(64) void k_batch_fftcc_mma(const PairDesc* __restri
 //   below 2^14), rounded to fp16, column 0 repacked as P[.][0] + i P[.][32].
 //   Inverse: Q = P^T F* for k = 0 .. 31 (scaled by 2^-8, rows k >= 1 doubled
 //   for their conjugates), row 0 giving Q[0][a] + i Q[32][a]; then C^T =
-//   Re(F* Q) + (-1)^b Q[32][a]: 144 MFMAs per POI against the integer
-//   engine's 512 of the same length.
+//   Re(F* Q) + (-1)^b Q[32][a], the last term through the imaginary table
+//   part of k = 0 (zero in F, (-1)^b here): 144 MFMAs per POI against the
+//   integer engine's 512 of the same length.
 //   Layouts: a 32 x 32 x 16 accumulator holds column n in lane n (+32 h)
 //   and rows 8 (r / 4) + 4 h + r % 4 in register r, so registers 8 c' .. 8 c'
 //   + 7 of a tile are a ready B (or, for a transposed product, A) operand
@@ -1174,15 +1175,23 @@ typedef _Float16 h2v __attribute__((ext_vector_type(2)));
 typedef float f16x __attribute__((ext_vector_type(16)));
 
 constexpr double kDftBound = 22.0;        // N^2 |C^ - C| / (||r|| ||t||), see above
-constexpr int kExR = 80, kExT = 144;      // LDS pitch of the R rows (64 B) and the doubled T rows (128 B)
+// LDS pitch of the R rows (64 B) and the doubled T rows (128 B): 18 and 34
+// dwords put the 8-byte operand reads of 16 consecutive rows on distinct banks
+constexpr int kExR = 72, kExT = 136;
 constexpr int kDftWaves = 8;              // POIs in flight per workgroup (one per wavefront)
 
 // F in fp16 as MFMA operands, [table][part][tile][K chunk][lane], lane = 32 h + m,
 // parts re, im, -im:
 //   table 0: F[16 c + 8 h + q][32 t + m]                 (B operand, K in natural order);
 //            in tile 0 the imaginary part of column 0 holds (-1)^x instead of 0
-//   table 1: F[32 t + m][16 c + 8 (q / 4) + 4 h + q % 4]   (A operand, K in accumulator row order)
+//   table 1: 2^-8 F[32 t + m][16 c + 8 (q / 4) + 4 h + q % 4]   (A operand, K in accumulator
+//            row order; the 2^-8, exact, keeps the first inverse pass's output in fp16 range)
 __device__ h8 g_dft_tab[2][3][2][4][64];
+// the last pass's imaginary table part for K chunk 0 with the k = 0 entry
+// (zero in F) set to 2^-8 (-1)^b: the Nyquist row Q[32][a], carried in the
+// imaginary part of row 0, then enters C^T[b][a] with its sign (-1)^b
+__device__ h8 g_dft_nyq[2][64];
+int g_dft_cus[64] = {};                        // compute units per device, set by dft_tables_init
 
 // One workgroup: the tables (32 KB, read by all its wavefronts) and, per
 // wavefront, its POI's R rows and doubled T rows as bytes and the second
@@ -1190,8 +1199,9 @@ __device__ h8 g_dft_tab[2][3][2][4][64];
 struct DftWgSmem {
     h8 ta[2][4][64];
     h8 tb[3][2][4][64];
+    h8 tn[2][64];
     uint8_t ex[kDftWaves][64 * kExR + 64 * kExT];
-    float4 col0[kDftWaves][64];
+    float col0[kDftWaves][4][64];               // Re R^, Im R^, Re T^, Im T^ of column 0, by row j
     uint32_t xcol[kDftWaves][64];
 };
 
@@ -1217,6 +1227,47 @@ __device__ __forceinline__ void wave_lds_sync()
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Wave-uniform reductions for the DFT engine: butterflies inside each row of
+// 16 lanes by DPP (xor 1, xor 2, half-row mirror, row mirror), then the four
+// row results by readlane.  Every lane must be active.
+template <class Op>
+__device__ __forceinline__ int wave_reduce_dpp(int v, Op op)
+{
+    v = op(v, __builtin_amdgcn_update_dpp(v, v, 0xB1, 0xF, 0xF, false));
+    v = op(v, __builtin_amdgcn_update_dpp(v, v, 0x4E, 0xF, 0xF, false));
+    v = op(v, __builtin_amdgcn_update_dpp(v, v, 0x141, 0xF, 0xF, false));
+    v = op(v, __builtin_amdgcn_update_dpp(v, v, 0x140, 0xF, 0xF, false));
+    const int a = __builtin_amdgcn_readlane(v, 0), b = __builtin_amdgcn_readlane(v, 16);
+    const int c = __builtin_amdgcn_readlane(v, 32), d = __builtin_amdgcn_readlane(v, 48);
+    return op(op(a, b), op(c, d));
+}
+__device__ __forceinline__ int dpp_sum(int v)
+{
+    return wave_reduce_dpp(v, [](int x, int y) { return x + y; });
+}
+__device__ __forceinline__ int dpp_min(int v)
+{
+    return wave_reduce_dpp(v, [](int x, int y) { return min(x, y); });
+}
+__device__ __forceinline__ float dpp_max(float v)
+{
+    return __int_as_float(wave_reduce_dpp(__float_as_int(v), [](int x, int y) {
+        return __float_as_int(fmaxf(__int_as_float(x), __int_as_float(y)));
+    }));
+}
+
+// 2 mask + (x >= thr): the compare's lane mask is the carry-in of an add,
+// two instructions per bit
+__device__ __forceinline__ uint32_t shift_in_ge(uint32_t mask, float x, float thr)
+{
+    uint32_t r;
+    asm("v_cmp_ge_f32_e32 vcc, %1, %2\n\tv_addc_co_u32_e32 %0, vcc, %3, %3, vcc"
+        : "=v"(r)
+        : "v"(x), "v"(thr), "v"(mask)
+        : "vcc");
+    return r;
 }
 
 // Eight window bytes as the fp16 values byte - off: the byte v becomes the
@@ -1251,7 +1302,7 @@ __device__ void fftcc_dft_mfma_poi(DftWgSmem& sm, int wave, int lane, const Fftc
     uint8_t* ex = sm.ex[wave];
     const uint8_t* erows = ex;                 // R row y at y kExR
     const uint8_t* etrows = ex + 64 * kExR;    // T row y at y kExT, twice
-    float4* col0 = sm.col0[wave];
+    float (*col0)[64] = sm.col0[wave];
     uint32_t* xcol = sm.xcol[wave];
     // -- window rows (lane = row) to LDS, exact integer sums -------------------
     int sr = 0, st = 0, srr = 0, stt = 0;
@@ -1268,18 +1319,18 @@ __device__ void fftcc_dft_mfma_poi(DftWgSmem& sm, int wave, int lane, const Fftc
             stt = (int)__builtin_amdgcn_udot4(tw[k], tw[k], (uint32_t)stt, false);
         }
         wave_lds_sync();                       // the previous POI's reads of this region are done
-        uint4* er = reinterpret_cast<uint4*>(ex + lane * kExR);
-        uint4* et = reinterpret_cast<uint4*>(ex + 64 * kExR + lane * kExT);
+        uint2* er = reinterpret_cast<uint2*>(ex + lane * kExR);
+        uint2* et = reinterpret_cast<uint2*>(ex + 64 * kExR + lane * kExT);
 #pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            const uint4 a = make_uint4(rw[4 * k], rw[4 * k + 1], rw[4 * k + 2], rw[4 * k + 3]);
-            const uint4 b = make_uint4(tw[4 * k], tw[4 * k + 1], tw[4 * k + 2], tw[4 * k + 3]);
+        for (int k = 0; k < 8; ++k) {
+            const uint2 a = make_uint2(rw[2 * k], rw[2 * k + 1]);
+            const uint2 b = make_uint2(tw[2 * k], tw[2 * k + 1]);
             er[k] = a;
             et[k] = b;
-            et[4 + k] = b;
+            et[8 + k] = b;
         }
     }
-    sr = wave_sum_i(sr); st = wave_sum_i(st); srr = wave_sum_i(srr); stt = wave_sum_i(stt);
+    sr = dpp_sum(sr); st = dpp_sum(st); srr = dpp_sum(srr); stt = dpp_sum(stt);
     const long long Sr = sr, St = st, Srr = srr, Stt = stt;
     if (Srr * NN == Sr * Sr || Stt * NN == St * St) {          // flat window (:271-275)
         if (lane == 0 && p.status) p.status[poi] = TWDIC_FFTCC_FLAT;
@@ -1294,7 +1345,7 @@ __device__ void fftcc_dft_mfma_poi(DftWgSmem& sm, int wave, int lane, const Fftc
     int ex2;
     (void)frexp(4096.0 * rho, &ex2);
     const float s = ldexpf(1.0f, 14 - ex2);                                      // s 4096 rho in [2^13, 2^14)
-    const float tol = (float)(2.0 * 1.25 * kDftBound * rho * (double)s / 256.0); // in C^ units (s 2^-8 N^2 C)
+    const float tol = (float)(2.0 * 1.25 * kDftBound * rho * (double)s / 65536.0); // in C^ units (s 2^-16 N^2 C)
     wave_lds_sync();
 
     // -- first pass, columns k = 0 .. 31 (column 0 = Y[.][0] + i Y[.][32]) -------
@@ -1341,8 +1392,13 @@ __device__ void fftcc_dft_mfma_poi(DftWgSmem& sm, int wave, int lane, const Fftc
         }
         if (m == 0) {                          // column 0: W_R = R^[j][0] + i R^[j][32], W_T likewise
 #pragma unroll
-            for (int r = 0; r < 16; ++r)
-                col0[32 * Mj + 8 * (r >> 2) + 4 * h + (r & 3)] = make_float4(Rr[r], Ri[r], Tr[r], Ti[r]);
+            for (int g = 0; g < 4; ++g) {      // registers 4 g .. 4 g + 3 hold rows j0 .. j0 + 3
+                const int j0 = 32 * Mj + 8 * g + 4 * h;
+                *reinterpret_cast<float4*>(&col0[0][j0]) = make_float4(Rr[4 * g], Rr[4 * g + 1], Rr[4 * g + 2], Rr[4 * g + 3]);
+                *reinterpret_cast<float4*>(&col0[1][j0]) = make_float4(Ri[4 * g], Ri[4 * g + 1], Ri[4 * g + 2], Ri[4 * g + 3]);
+                *reinterpret_cast<float4*>(&col0[2][j0]) = make_float4(Tr[4 * g], Tr[4 * g + 1], Tr[4 * g + 2], Tr[4 * g + 3]);
+                *reinterpret_cast<float4*>(&col0[3][j0]) = make_float4(Ti[4 * g], Ti[4 * g + 1], Ti[4 * g + 2], Ti[4 * g + 3]);
+            }
         }
         f16x pr, pi;
 #pragma unroll
@@ -1350,10 +1406,12 @@ __device__ void fftcc_dft_mfma_poi(DftWgSmem& sm, int wave, int lane, const Fftc
             pr[i] = fmaf(Rr[i], Tr[i], Ri[i] * Ti[i]);
             pi[i] = fmaf(Rr[i], Ti[i], -(Ri[i] * Tr[i]));
         }
+        // columns k >= 1 doubled for their conjugates 64 - k (column 0 is
+        // rewritten below); 2^16 undoes the two 2^-8 of the table
 #pragma unroll
         for (int hf = 0; hf < 2; ++hf) {
-            Pre[2 * Mj + hf] = to_h8(pr, hf, s);
-            Pim[2 * Mj + hf] = to_h8(pi, hf, s);
+            Pre[2 * Mj + hf] = to_h8(pr, hf, 131072.f * s);
+            Pim[2 * Mj + hf] = to_h8(pi, hf, 131072.f * s);
         }
         __builtin_amdgcn_sched_barrier(0);
     }
@@ -1363,14 +1421,16 @@ __device__ void fftcc_dft_mfma_poi(DftWgSmem& sm, int wave, int lane, const Fftc
     // collect theirs
     wave_lds_sync();
     {
-        const float4 w = col0[lane], wm = col0[(64 - lane) & 63];
+        const int jm = (64 - lane) & 63;
+        const float4 w = make_float4(col0[0][lane], col0[1][lane], col0[2][lane], col0[3][lane]);
+        const float4 wm = make_float4(col0[0][jm], col0[1][jm], col0[2][jm], col0[3][jm]);
         const float ar = 0.5f * (w.x + wm.x), ai = 0.5f * (w.y - wm.y);    // R^[j][0]
         const float br = 0.5f * (w.y + wm.y), bi = 0.5f * (wm.x - w.x);    // R^[j][32]
         const float cr = 0.5f * (w.z + wm.z), ci = 0.5f * (w.w - wm.w);    // T^[j][0]
         const float dr = 0.5f * (w.w + wm.w), di = 0.5f * (wm.z - w.z);    // T^[j][32]
         const float p0r = fmaf(ar, cr, ai * ci), p0i = fmaf(ar, ci, -(ai * cr));   // conj(R^0) T^0
         const float p1r = fmaf(br, dr, bi * di), p1i = fmaf(br, di, -(bi * dr));   // conj(R^32) T^32
-        const h2v x = {(_Float16)((p0r - p1i) * s), (_Float16)((p0i + p1r) * s)};
+        const h2v x = {(_Float16)((p0r - p1i) * (65536.f * s)), (_Float16)((p0i + p1r) * (65536.f * s))};
         xcol[lane] = __builtin_bit_cast(uint32_t, x);
     }
     wave_lds_sync();
@@ -1387,7 +1447,6 @@ __device__ void fftcc_dft_mfma_poi(DftWgSmem& sm, int wave, int lane, const Fftc
     // -- first inverse pass, rows k = 0 .. 31: Q[k][a] = sum_j P[j][k] conj(F[j][a])
     //    (row 0 comes out as Q[0][a] + i Q[32][a], both real) -----------------------
     h8 Qr[2][2], Qi[2][2];                     // B operands of the last pass [k chunk][a tile]
-    float q32[2];                              // Q[32][a] of this lane's column a, per a tile
     {
         f16x qr[2] = {}, qi[2] = {};
 #pragma unroll
@@ -1403,22 +1462,10 @@ __device__ void fftcc_dft_mfma_poi(DftWgSmem& sm, int wave, int lane, const Fftc
         }
 #pragma unroll
         for (int Nc = 0; Nc < 2; ++Nc) {
-            // row k = 0 is register 0 of the lanes h = 0
-            q32[Nc] = __shfl(qi[Nc][0], m) * (1.f / 256.f);
-            // the last pass sums k = 0 .. 31 with weight 1 for k = 0 and 2 for
-            // the others (the conjugate rows 33 .. 63)
-            const float w0 = h == 0 ? 1.f / 256.f : 2.f / 256.f;
 #pragma unroll
             for (int hf = 0; hf < 2; ++hf) {
-                h8 vr, vi;
-#pragma unroll
-                for (int q = 0; q < 8; ++q) {
-                    const float sc = (hf == 0 && q == 0) ? w0 : 2.f / 256.f;
-                    vr[q] = (_Float16)(qr[Nc][8 * hf + q] * sc);
-                    vi[q] = (_Float16)(qi[Nc][8 * hf + q] * sc);
-                }
-                Qr[hf][Nc] = vr;
-                Qi[hf][Nc] = vi;
+                Qr[hf][Nc] = to_h8(qr[Nc], hf, 1.f);
+                Qi[hf][Nc] = to_h8(qi[Nc], hf, 1.f);
             }
         }
         __builtin_amdgcn_sched_barrier(0);
@@ -1429,7 +1476,7 @@ __device__ void fftcc_dft_mfma_poi(DftWgSmem& sm, int wave, int lane, const Fftc
     for (int c = 0; c < 2; ++c) {
 #pragma unroll
         for (int Md = 0; Md < 2; ++Md) {
-            const h8 Fr = sm.tb[0][Md][c][lane], Fi = sm.tb[1][Md][c][lane];
+            const h8 Fr = sm.tb[0][Md][c][lane], Fi = c == 0 ? sm.tn[Md][lane] : sm.tb[1][Md][c][lane];
 #pragma unroll
             for (int Nd = 0; Nd < 2; ++Nd) {
                 Cd[Md][Nd] = mfma16(Fr, Qr[c][Nd], Cd[Md][Nd]);
@@ -1437,34 +1484,36 @@ __device__ void fftcc_dft_mfma_poi(DftWgSmem& sm, int wave, int lane, const Fftc
             }
         }
     }
-#pragma unroll
-    for (int Md = 0; Md < 2; ++Md)
-#pragma unroll
-        for (int Nd = 0; Nd < 2; ++Nd)
-#pragma unroll
-            for (int r = 0; r < 16; ++r) Cd[Md][Nd][r] += (r & 1) ? -q32[Nd] : q32[Nd];
 
     // -- candidates: every shift within tol of the approximate maximum -----------
-    float mx = -FLT_MAX;
+    float gmx[2][2];                           // per 16-register group [Nd][Md]
 #pragma unroll
     for (int Md = 0; Md < 2; ++Md)
 #pragma unroll
-        for (int Nd = 0; Nd < 2; ++Nd)
+        for (int Nd = 0; Nd < 2; ++Nd) {
+            float g = Cd[Md][Nd][0];
 #pragma unroll
-            for (int r = 0; r < 16; ++r) mx = fmaxf(mx, Cd[Md][Nd][r]);
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) mx = fmaxf(mx, __shfl_xor(mx, off));
-    const float thr = mx - tol;
-    // bit 16 (2 Nd + Md) + r: this lane's shifts in increasing row-major index a N + b
-    uint32_t mlo = 0, mhi = 0;
-#pragma unroll
-    for (int Md = 0; Md < 2; ++Md)
-#pragma unroll
-        for (int r = 0; r < 16; ++r) {
-            mlo |= (uint32_t)(Cd[Md][0][r] >= thr) << (16 * Md + r);
-            mhi |= (uint32_t)(Cd[Md][1][r] >= thr) << (16 * Md + r);
+            for (int r = 1; r < 16; ++r) g = fmaxf(g, Cd[Md][Nd][r]);
+            gmx[Nd][Md] = g;
         }
-    uint64_t mask = ((uint64_t)mhi << 32) | mlo;
+    const float mx = dpp_max(fmaxf(fmaxf(gmx[0][0], gmx[0][1]), fmaxf(gmx[1][0], gmx[1][1])));
+    const float thr = mx - tol;
+    // bit 16 (2 Nd + Md) + r: this lane's shifts in increasing row-major index a
+    // N + b (shifted in from the top bit down); a group no lane has a candidate
+    // in is skipped
+    uint32_t mw[2] = {0u, 0u};
+#pragma unroll
+    for (int Nd = 0; Nd < 2; ++Nd)
+#pragma unroll
+        for (int Md = 1; Md >= 0; --Md) {
+            uint32_t bits = 0;
+            if (gmx[Nd][Md] >= thr) {
+#pragma unroll
+                for (int r = 15; r >= 0; --r) bits = shift_in_ge(bits, Cd[Md][Nd][r], thr);
+            }
+            mw[Nd] |= bits << (16 * Md);
+        }
+    uint64_t mask = ((uint64_t)mw[1] << 32) | mw[0];
 
     // -- exact integer correlation at the candidates, in increasing index order ---
     const uint32_t* rrow = reinterpret_cast<const uint32_t*>(erows + lane * kExR);
@@ -1476,7 +1525,7 @@ __device__ void fftcc_dft_mfma_poi(DftWgSmem& sm, int wave, int lane, const Fftc
             const int Nd = bit >> 5, Md = (bit >> 4) & 1, r = bit & 15;
             mine = (32 * Nd + m) * N + 32 * Md + 8 * (r >> 2) + 4 * h + (r & 3);
         }
-        const int k = wave_min_i(mine);
+        const int k = dpp_min(mine);
         if (k == INT_MAX) break;
         const int kv = k >> 6, ku = k & 63, d = ku >> 2, sh = ku & 3;
         const uint32_t* trow = reinterpret_cast<const uint32_t*>(etrows + ((lane + kv) & 63) * kExT);
@@ -1488,7 +1537,7 @@ __device__ void fftcc_dft_mfma_poi(DftWgSmem& sm, int wave, int lane, const Fftc
             acc = __builtin_amdgcn_udot4(rrow[j], __builtin_amdgcn_alignbyte(w1, w0, sh), acc, false);
             w0 = w1;
         }
-        const int I = wave_sum_i((int)acc);
+        const int I = dpp_sum((int)acc);
         if (I > bestI) {
             bestI = I;
             peak = k;
@@ -1508,27 +1557,66 @@ __device__ void fftcc_dft_mfma_poi(DftWgSmem& sm, int wave, int lane, const Fftc
 }
 
 // Workgroups of kDftWaves wavefronts share the DFT tables in LDS; workgroup
-// b takes the contiguous POI range [b n / G, (b + 1) n / G) (G workgroups),
-// its wavefronts every kDftWaves-th POI of it.
+// b takes the contiguous POI range [g n / G, (g + 1) n / G) (G workgroups),
+// its wavefronts every kDftWaves-th POI of it.  Workgroups go to the 8 XCDs
+// round-robin, so g = (b mod 8) G / 8 + b / 8 when 8 divides G: each XCD
+// (and its L2) then covers one contiguous eighth of the POIs, a band of rows.
+__device__ __forceinline__ void dft_tables_to_lds(DftWgSmem& sm)
+{
+    const h8* gb = &g_dft_tab[1][0][0][0][0];
+    h8* la = &sm.ta[0][0][0];
+    h8* lb = &sm.tb[0][0][0][0];
+    for (int i = threadIdx.x; i < 2 * 4 * 64; i += blockDim.x)        // table 0, tile 0, parts re and im
+        la[i] = g_dft_tab[0][i >> 8][0][(i >> 6) & 3][i & 63];
+    for (int i = threadIdx.x; i < 3 * 2 * 4 * 64; i += blockDim.x) lb[i] = gb[i];
+    for (int i = threadIdx.x; i < 2 * 64; i += blockDim.x) sm.tn[i >> 6][i & 63] = g_dft_nyq[i >> 6][i & 63];
+    __syncthreads();
+}
+
+__device__ __forceinline__ int dft_range_index()
+{
+    const int G = gridDim.x, b = blockIdx.x;
+    return (G & 7) == 0 ? (b & 7) * (G >> 3) + (b >> 3) : b;
+}
+
 __global__ __launch_bounds__(64 * kDftWaves, 1) void k_fftcc_dft_mfma(FftccParams p)
 {
     __shared__ DftWgSmem sm;
-    {
-        const h8* gb = &g_dft_tab[1][0][0][0][0];
-        h8* la = &sm.ta[0][0][0];
-        h8* lb = &sm.tb[0][0][0][0];
-        for (int i = threadIdx.x; i < 2 * 4 * 64; i += blockDim.x)        // table 0, tile 0, parts re and im
-            la[i] = g_dft_tab[0][i >> 8][0][(i >> 6) & 3][i & 63];
-        for (int i = threadIdx.x; i < 3 * 2 * 4 * 64; i += blockDim.x) lb[i] = gb[i];
-    }
-    __syncthreads();
+    dft_tables_to_lds(sm);
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int lo = (int)((long long)blockIdx.x * p.n / gridDim.x);
-    const int hi = (int)((long long)(blockIdx.x + 1) * p.n / gridDim.x);
+    const int g = dft_range_index();
+    const int lo = (int)((long long)g * p.n / gridDim.x);
+    const int hi = (int)((long long)(g + 1) * p.n / gridDim.x);
     for (int poi = lo + wave; poi < hi; poi += kDftWaves) fftcc_dft_mfma_poi(sm, wave, lane, p, poi);
 }
 
-int g_dft_cus[64] = {};                        // compute units per device, set by dft_tables_init
+// Batched form over items (pair, POI) of pairs with N = 64: the same ranges
+// per workgroup, each wavefront reading its item's pair descriptor.
+__global__ __launch_bounds__(64 * kDftWaves, 1) void k_batch_fftcc_dft_mfma(const PairDesc* __restrict__ pairs,
+                                                                         const int2* __restrict__ items, int count)
+{
+    __shared__ DftWgSmem sm;
+    dft_tables_to_lds(sm);
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int g = dft_range_index();
+    const int lo = (int)((long long)g * count / gridDim.x);
+    const int hi = (int)((long long)(g + 1) * count / gridDim.x);
+    for (int i = lo + wave; i < hi; i += kDftWaves) {
+        const int2 it = items[i];
+        const FftccParams p = const_pairs(pairs)[__builtin_amdgcn_readfirstlane(it.x)].fftcc;
+        fftcc_dft_mfma_poi(sm, wave, lane, p, __builtin_amdgcn_readfirstlane(it.y));
+    }
+}
+
+// workgroups for n POIs: one per compute unit (each holds one), fewer for
+// small n
+int dft_blocks(int n)
+{
+    int dev = 0;
+    (void)hipGetDevice(&dev);
+    const int cus = dev >= 0 && dev < 64 && g_dft_cus[dev] > 0 ? g_dft_cus[dev] : 256;
+    return std::max(1, std::min(cus, (n + kDftWaves - 1) / kDftWaves));
+}
 
 }  // namespace
 
@@ -1541,7 +1629,10 @@ hipError_t launch_batch_fftcc_mma(const PairDesc* pairs, const int2* items, int 
 {
     if (count <= 0) return hipSuccess;
     switch (form) {
-    case 0: hipLaunchKernelGGL((k_batch_fftcc_mma<64, true, 32>), dim3(count), dim3(64), 0, st, pairs, items); break;
+    case 0:
+        hipLaunchKernelGGL(k_batch_fftcc_dft_mfma, dim3(dft_blocks(count)), dim3(64 * kDftWaves), 0, st, pairs, items,
+                           count);
+        break;
     case 1: hipLaunchKernelGGL((k_batch_fftcc_mma<64, false, 32>), dim3(count), dim3(64), 0, st, pairs, items); break;
     case 2: hipLaunchKernelGGL((k_batch_fftcc_mma<64, false, 16>), dim3(count), dim3(64), 0, st, pairs, items); break;
     default: hipLaunchKernelGGL((k_batch_fftcc_mma<32, false, 32>), dim3(count), dim3(64), 0, st, pairs, items); break;
@@ -1556,11 +1647,7 @@ hipError_t launch_fftcc(const FftccParams& p, int variant, hipStream_t st)
     if (p.n <= 0) return hipSuccess;
     const int pairs = (p.n + 1) / 2;
     if (variant == kFftccDftMfma && p.N == 64) {
-        int dev = 0;
-        (void)hipGetDevice(&dev);
-        const int cus = dev >= 0 && dev < 64 && g_dft_cus[dev] > 0 ? g_dft_cus[dev] : 256;
-        const int blocks = std::max(1, std::min(cus, (p.n + kDftWaves - 1) / kDftWaves));
-        hipLaunchKernelGGL(k_fftcc_dft_mfma, dim3(blocks), dim3(64 * kDftWaves), 0, st, p);
+        hipLaunchKernelGGL(k_fftcc_dft_mfma, dim3(dft_blocks(p.n)), dim3(64 * kDftWaves), 0, st, p);
     } else if (variant == kFftccMma || variant == kFftccDftMfma) {
         if (p.N == 64)
             hipLaunchKernelGGL((k_fftcc_mma<64, true, 32>), dim3(p.n), dim3(64), 0, st, p);
@@ -1637,17 +1724,29 @@ hipError_t dft_tables_init(int device)
                         const int row = t == 0 ? 16 * c + 8 * h + q : 32 * tile + m;
                         const int col = t == 0 ? 32 * tile + m : 16 * c + 8 * (q / 4) + 4 * h + q % 4;
                         const double ang = -2.0 * M_PI * (double)((row * col) & 63) / 64.0;
-                        host[t][0][tile][c][lane][q] = (_Float16)std::cos(ang);   // direct double -> fp16, nearest-even
-                        host[t][1][tile][c][lane][q] = (_Float16)std::sin(ang);
+                        // direct double -> fp16, nearest-even; table 1 then times 2^-8 (exact:
+                        // its non-zero entries stay normal)
+                        const double sc = t == 0 ? 1.0 : 1.0 / 256.0;
+                        host[t][0][tile][c][lane][q] = (_Float16)((double)(_Float16)std::cos(ang) * sc);
+                        host[t][1][tile][c][lane][q] = (_Float16)((double)(_Float16)std::sin(ang) * sc);
                         if (t == 0 && tile == 0 && m == 0)                        // (-1)^x: Y[.][32]
                             host[t][1][tile][c][lane][q] = (_Float16)(((16 * c + 8 * h + q) & 1) ? -1.0 : 1.0);
                         host[t][2][tile][c][lane][q] = -host[t][1][tile][c][lane][q];
                     }
+    static _Float16 nyq[2][64][8];
+    for (int tile = 0; tile < 2; ++tile)
+        for (int lane = 0; lane < 64; ++lane)
+            for (int q = 0; q < 8; ++q) {
+                nyq[tile][lane][q] = host[1][1][tile][0][lane][q];
+                if ((lane >> 5) == 0 && q == 0)                                   // K index 0: b = 32 tile + m
+                    nyq[tile][lane][q] = (_Float16)((lane & 1) ? -1.0 / 256.0 : 1.0 / 256.0);
+            }
     int cur = 0;
     hipError_t e = hipGetDevice(&cur);
     if (e == hipSuccess) e = hipDeviceGetAttribute(&g_dft_cus[device], hipDeviceAttributeMultiprocessorCount, device);
     if (e == hipSuccess) e = hipSetDevice(device);
     if (e == hipSuccess) e = hipMemcpyToSymbol(HIP_SYMBOL(g_dft_tab), host, sizeof(host));
+    if (e == hipSuccess) e = hipMemcpyToSymbol(HIP_SYMBOL(g_dft_nyq), nyq, sizeof(nyq));
     if (e == hipSuccess) e = hipSetDevice(cur);
     if (e == hipSuccess) done[device] = true;
     return e;

@@ -74,19 +74,42 @@ __device__ __forceinline__ float row_dot(uint32_t w, const float wx[4])
     return fmaf(wx[3], (float)(w >> 24), s);
 }
 
+// The target image as a buffer resource whose base is the pixel (-1, -1):
+// the 4x4 neighbourhood of (ix, iy) is then four unaligned dwords at byte
+// offset iy stride + ix, plus 0, stride, 2 stride and 3 stride in the scalar
+// offset, and the pixel itself is at that offset plus stride + 1.  ix, iy >= 1
+// (the ROI keeps a 1-pixel frame inside the image wherever IC-GN runs), so
+// every address lies in the image; offsets are 32-bit (images below 2 GiB).
+// The range check (vector plus scalar offset below the ROI's last row + 4
+// rows from the base) turns a stray offset into zeros instead of a fault.
+struct TargetBuf {
+    __amdgpu_buffer_rsrc_t r;
+    int stride;
+};
+
+__device__ __forceinline__ TargetBuf target_buf(const IcgnParams& P)
+{
+    const int stride = P.stride;
+    const uint8_t* base = P.tar - stride - 1;
+    const int records = (P.sy0 + P.rh + 3) * stride;
+    return {__builtin_amdgcn_make_buffer_rsrc((void*)base, (short)0, records, 0x00020000), stride};
+}
+
+__device__ __forceinline__ int target_offset(const TargetBuf& B, int ix, int iy)
+{
+    return (int)__umul24((uint32_t)iy, (uint32_t)B.stride) + ix;
+}
+
 // One unaligned dword per neighbourhood row.  A uniform neighbourhood returns
 // its value exactly (as the LUT does: all higher coefficients vanish), so
 // uniform target regions keep the reference's DarkSubset behaviour.
-// ix, iy >= 1 (interior pixels); offsets are 32-bit, so images are below 4 GiB.
-__device__ __forceinline__ float interp_fused(const uint8_t* __restrict__ tar, uint32_t stride, int ix, int iy,
-                                              float tx, float ty)
+__device__ __forceinline__ float interp_fused(const TargetBuf& B, int ix, int iy, float tx, float ty)
 {
-    const uint32_t o0 = __umul24((uint32_t)(iy - 1), stride) + (uint32_t)(ix - 1);
-    uint32_t r0, r1, r2, r3;
-    __builtin_memcpy(&r0, tar + o0, 4);
-    __builtin_memcpy(&r1, tar + (o0 + stride), 4);
-    __builtin_memcpy(&r2, tar + (o0 + 2 * stride), 4);
-    __builtin_memcpy(&r3, tar + (o0 + 3 * stride), 4);
+    const int o = target_offset(B, ix, iy);
+    const uint32_t r0 = __builtin_amdgcn_raw_buffer_load_b32(B.r, o, 0, 0);
+    const uint32_t r1 = __builtin_amdgcn_raw_buffer_load_b32(B.r, o, B.stride, 0);
+    const uint32_t r2 = __builtin_amdgcn_raw_buffer_load_b32(B.r, o, 2 * B.stride, 0);
+    const uint32_t r3 = __builtin_amdgcn_raw_buffer_load_b32(B.r, o, 3 * B.stride, 0);
     float wx[4], wy[4];
     catmull_rom_w(tx, wx);
     catmull_rom_w(ty, wy);
@@ -95,10 +118,11 @@ __device__ __forceinline__ float interp_fused(const uint8_t* __restrict__ tar, u
     acc = fmaf(wy[2], row_dot(r2, wx), acc);
     acc = fmaf(wy[3], row_dot(r3, wx), acc);
     // uniform: rows 0, 2, 3 equal row 1, and row 1 equals itself turned by a
-    // byte.  XORs and ORs issue beside the FP32 work; kept opaque, or the
-    // compiler splits the test back into four compares, which do not.
-    uint32_t diff = (r0 ^ r1) | (r2 ^ r1) | (r3 ^ r1) | (r1 ^ __builtin_amdgcn_alignbit(r1, r1, 8));
-    asm volatile("" : "+v"(diff));
+    // byte: a chain of byte absolute-difference sums is zero
+    uint32_t diff = __builtin_amdgcn_sad_u8(r0, r1, 0u);
+    diff = __builtin_amdgcn_sad_u8(r2, r1, diff);
+    diff = __builtin_amdgcn_sad_u8(r3, r1, diff);
+    diff = __builtin_amdgcn_sad_u8(r1, __builtin_amdgcn_alignbit(r1, r1, 8), diff);
     return diff == 0u ? (float)((r1 >> 8) & 255u) : acc;
 }
 
@@ -177,12 +201,13 @@ __device__ __forceinline__ bool subset_loop(const IcgnParams& P, int px, int py,
 {
     const int Sx = P.Sx, Sy = P.Sy, Wsub = P.Wsub, s = P.s;
     const int sx0 = P.sx0, sy0 = P.sy0, rw = P.rw, rh = P.rh;
-    const uint32_t stride = (uint32_t)P.stride;
-    const uint8_t* g2b = reinterpret_cast<const uint8_t*>(P.g2);
+    const TargetBuf tb = target_buf(P);
+    const __amdgpu_buffer_rsrc_t g2r =
+        __builtin_amdgcn_make_buffer_rsrc((void*)P.g2, (short)0, 4 * rw * rh, 0x00020000);
     const float fpx = (float)px, fpy = (float)py;
     double st = 0.0, stt = 0.0, a0 = 0.0, a1 = 0.0, a2 = 0.0, b0 = 0.0, b1 = 0.0, b2 = 0.0;
     bool out = false;
-    auto pixel = [&](float dx, float dy, uint32_t go) {
+    auto pixel = [&](float dx, float dy, uint32_t go, int goff) {
         const float wx = warp_coord(fpx, Wm[0], Wm[1], Wm[2], dx, dy);
         const float wy = warp_coord(fpy, Wm[3], Wm[4], Wm[5], dx, dy);
         const int ix = (int)wx, iy = (int)wy;          // truncation, as int()
@@ -195,16 +220,16 @@ __device__ __forceinline__ bool subset_loop(const IcgnParams& P, int px, int py,
             out |= !in;
         }
         if (INT) {
-            if (in) T = (float)P.tar[__umul24((uint32_t)iy, stride) + (uint32_t)ix];     // :629
+            if (in) T = (float)__builtin_amdgcn_raw_buffer_load_b8(tb.r, target_offset(tb, ix, iy), tb.stride + 1, 0); // :629
         } else if (interior) {
             // :771-772; wx, wy >= 1 here, so x - (float)(int)x is fract(x), exactly
             const float tx = __builtin_amdgcn_fractf(wx), ty = __builtin_amdgcn_fractf(wy);
             if (INTERP == TWDIC_INTERP_FUSED)
-                T = interp_fused(P.tar, stride, ix, iy, tx, ty);
+                T = interp_fused(tb, ix, iy, tx, ty);
             else
                 T = interp_lut(P.lut, (uint32_t)((iy - sy0) * rw + (ix - sx0)), tx, ty);
         }
-        const uint32_t g = *reinterpret_cast<const uint32_t*>(g2b + go);
+        const uint32_t g = __builtin_amdgcn_raw_buffer_load_b32(g2r, (int)go, goff, 0);
         // FP64: every product SD_k T is exact, and where T takes integer values
         // (the integer pass, uniform target regions) the sums are exact, so
         // T = R gives an exactly zero update and near-flat subsets, whose
@@ -222,19 +247,19 @@ __device__ __forceinline__ bool subset_loop(const IcgnParams& P, int px, int py,
         b2 = fma(b, ddy, b2);
     };
     // byte offset of the subset's corner pixel in the gradient plane
-    const uint32_t gcorner = 4u * (uint32_t)((py - Sy - sy0) * rw + (px - Sx - sx0));
+    const int gcorner = 4 * ((py - Sy - sy0) * rw + (px - Sx - sx0));
     if constexpr (TABLE) {
         const int full = s >> 6;
 #pragma unroll 1
         for (int k = 0; k < full; ++k) {
             const int p = lane + 64 * k;
             const float2 d = tab->d[p];
-            pixel(d.x, d.y, gcorner + tab->go[p]);
+            pixel(d.x, d.y, tab->go[p], gcorner);
         }
         const int p = lane + 64 * full;
         if (p < s) {
             const float2 d = tab->d[p];
-            pixel(d.x, d.y, gcorner + tab->go[p]);
+            pixel(d.x, d.y, tab->go[p], gcorner);
         }
     } else {
         // lane walks pixels p = lane, lane + 64, ...: column m = p % Wsub, row
@@ -247,12 +272,12 @@ __device__ __forceinline__ bool subset_loop(const IcgnParams& P, int px, int py,
         asm volatile("" : "+v"(lane_v));
         const int l0 = lane_v / Wsub, m0 = lane_v - l0 * Wsub;
         float dx = (float)(m0 - Sx), dy = (float)(l0 - Sy);
-        uint32_t go = gcorner + 4u * (uint32_t)(l0 * rw + m0);
+        uint32_t go = (uint32_t)gcorner + 4u * (uint32_t)(l0 * rw + m0);
         const float fSx = (float)Sx, fWsub = (float)Wsub, fr64 = (float)r64, fq64 = (float)q64;
         const uint32_t gstep = 4u * (uint32_t)(q64 * rw + r64), gwrap = 4u * (uint32_t)(rw - Wsub);
 #pragma unroll 1
         for (int p = lane; p < s; p += 64) {
-            pixel(dx, dy, go);
+            pixel(dx, dy, go, 0);
             // next pixel of this lane: 64 further in row-major subset order
             dx += fr64;
             const bool wrap = dx > fSx;
@@ -439,9 +464,9 @@ __global__ __launch_bounds__(256, MINW > 0 ? MINW : 1) void k_batch_icgn_wg(cons
     }
 }
 
-bool step_table_fits(int s)
+bool step_table_fits(int wsub, int hsub)
 {
-    return s <= kStepTable;
+    return wsub * hsub <= kStepTable;
 }
 
 template <int MINW, bool TABLE, bool XCD>

@@ -370,7 +370,7 @@ bool admit(twdic_server* s, Slot& sl, Job& j)
     fp.poi = d_poi; fp.n = n; fp.N = r.fft_n; fp.tw = twiddles(s, r.fft_n); fp.tol = 1e-4f;
     fp.u = d_u; fp.v = d_v; fp.zncc = d_z; fp.status = nullptr;
     if (!fp.tw) chk(hipErrorOutOfMemory);
-    if (err == hipSuccess) chk(launch_fftcc(fp, kFftccMma, sl.st));
+    if (err == hipSuccess) chk(launch_fftcc(fp, kFftccDftMfma, sl.st));
     if (err == hipSuccess) chk(launch_icgn(ip, TWDIC_INTERP_FUSED, 5, sl.st));
     chk(hipMemcpyAsync(sl.h_out, d_out, out_bytes(n), hipMemcpyDeviceToHost, sl.st));
     chk(hipEventRecord(sl.done, sl.st));
@@ -461,7 +461,7 @@ twdic_status twdic_server_create(const twdic_server_config* cfg, twdic_server** 
         twdic_server_destroy(s);
         return st;
     };
-    if (hipSetDevice(cfg->device) != hipSuccess) return bad(TWDIC_EHIP);
+    if (hipSetDevice(cfg->device) != hipSuccess || dft_tables_init(cfg->device) != hipSuccess) return bad(TWDIC_EHIP);
     size_t budget = cfg->hbm_budget;
     if (!budget) {
         size_t fr = 0, tot = 0;

@@ -128,7 +128,7 @@ hipError_t launch_batch_fftcc_mma(const PairDesc* pairs, const int2* items, int 
 // batched form: IC-GN (fused interpolation) over workgroup items (pair, first
 // POI), four POIs of one pair per workgroup; `table` when every listed pair's
 // subset fits the step table (step_table_fits).
-bool step_table_fits(int s);
+bool step_table_fits(int wsub, int hsub);
 // waves: occupancy target 5 or 6; xcd_ranges: contiguous per-XCD item ranges
 hipError_t launch_batch_icgn_wg(const PairDesc* pairs, const int2* wg, int count, bool table, int waves,
                                 bool xcd_ranges, hipStream_t st);
@@ -183,7 +183,9 @@ struct twdic_ctx {
     uint64_t refstate_version = ~0ull;
     const int32_t* refstate_poi = nullptr;
     int refstate_n = -1;
-    int fftcc_variant = twdic::kFftccMma;  // FFT-CC engine (TWDIC_FFTCC_VARIANT: 8 matrix cores; 2-4 FFT)
+    // FFT-CC engine (TWDIC_FFTCC_VARIANT): 9 fp16 matrix-core DFTs for N = 64 (else
+    // 8), 8 exact integer correlation on the matrix cores, 2-4 FFT registers
+    int fftcc_variant = twdic::kFftccDftMfma;
     int icgn_variant = 5;             // occupancy target of the IC-GN kernel (0: compiler's choice)
     uint8_t* h_stage = nullptr;       // pinned staging for host images
     size_t h_stage_bytes = 0;
