@@ -105,26 +105,37 @@ class BatchPair:
         return (1, 1, self.size - 2, self.size - 2)
 
 
+def batch_layout(count=64, seed=1003) -> list[dict]:
+    """The batched configuration without its images: per pair the size, S,
+    grid step, N, affine field and POI array that batch_pairs draws (the same
+    random stream), for tests that need only the work lists."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(count):
+        size = int(rng.choice([512, 1024, 2048, 4096]))
+        S = int(rng.choice([10, 15, 20, 30]))
+        g = int(rng.choice([4, 5, 10]))
+        N = 32 if S <= 15 else 64
+        d = api.affine(*rng.uniform(-8, 8, 2), *rng.uniform(-0.01, 0.01, 3), size / 2, size / 2)
+        m = max(N // 2, S + 2 + int(np.ceil(8 + 0.01 * size)))
+        ys = np.arange(m, size - m, g)
+        yy, xx = np.meshgrid(ys, ys, indexing="ij")
+        poi = np.stack([yy.ravel(), xx.ravel()], 1).astype(np.int32)
+        out.append(dict(size=size, S=S, grid=g, N=N, deform=d, poi=poi))
+    return out
+
+
 def batch_pairs(count=64, seed=1003, on_pair=None) -> list[BatchPair]:
     """The batched configuration: `count` speckle pairs with sizes drawn from
     {512, 1024, 2048, 4096}^2, subset radius from {10, 15, 20, 30}, grid step
     from {4, 5, 10}, strains uniform in [-1 %, 1 %] and translations uniform in
     [-8, 8] px about the image centre; pair i's images from seed 1003000 + i.
     on_pair(i, pair) is called after each pair is made (progress)."""
-    rng = np.random.default_rng(seed)
     out = []
-    for i in range(count):
-        size = int(rng.choice([512, 1024, 2048, 4096]))
-        S = int(rng.choice([10, 15, 20, 30]))
-        g = int(rng.choice([4, 5, 10]))
-        N = 32 if S <= 15 else 64
-        d = api.affine(*rng.uniform(-8, 8, 2), *rng.uniform(-0.01, 0.01, 3), size / 2, size / 2)
-        ref, tar = api.synth_pair(size, size, int(0.015 * size * size), 1003000 + i, d, noise_sigma=2.0)
-        m = max(N // 2, S + 2 + int(np.ceil(8 + 0.01 * size)))
-        ys = np.arange(m, size - m, g)
-        yy, xx = np.meshgrid(ys, ys, indexing="ij")
-        poi = np.stack([yy.ravel(), xx.ravel()], 1).astype(np.int32)
-        out.append(BatchPair(size, S, g, N, ref, tar, poi, d))
+    for i, L in enumerate(batch_layout(count, seed)):
+        size = L["size"]
+        ref, tar = api.synth_pair(size, size, int(0.015 * size * size), 1003000 + i, L["deform"], noise_sigma=2.0)
+        out.append(BatchPair(size, L["S"], L["grid"], L["N"], ref, tar, L["poi"], L["deform"]))
         if on_pair:
             on_pair(i, out[-1])
     return out

@@ -35,6 +35,17 @@ HIP_FLAGS = [
     f"-I{INCLUDE}", f"-I{CSRC}",
 ]
 CXX_FLAGS = ["-O3", "-std=c++17", "-fPIC", "-Wall", "-fopenmp", f"-I{INCLUDE}", f"-I{CSRC}"]
+# host-code sanitizers for tw_host_check_asan: AddressSanitizer and
+# UndefinedBehaviorSanitizer on the host side only (the GPU code is built as
+# in the library); each -fsanitize= follows -Xarch_host, and the link line
+# says -fno-gpu-sanitize
+SAN_COMPILE = ["-O1", "-g", "-fno-omit-frame-pointer",
+               "-Xarch_host", "-fsanitize=address",
+               "-Xarch_host", "-fsanitize=undefined",
+               "-Xarch_host", "-fno-sanitize-recover=undefined"]
+SAN_LINK = ["-fno-gpu-sanitize", "-fsanitize=address", "-fsanitize=undefined"]
+ASAN_DIR = OBJ_DIR / "asan"
+HOST_CHECK = OUT_DIR / "tw_host_check_asan"
 
 
 def _sources():
@@ -64,6 +75,10 @@ def build(verbose: bool = False) -> Path:
     for src in host:
         obj = OBJ_DIR / (src.stem + ".o")
         jobs.append(["g++", *CXX_FLAGS, "-c", str(src), "-o", str(obj)])
+    ASAN_DIR.mkdir(parents=True, exist_ok=True)
+    san_srcs = hip_host + [CPPSRC / "examples" / "host_check.cpp"]
+    for src in san_srcs:
+        jobs.append([HIPCC, *HIP_FLAGS, *SAN_COMPILE, "-c", str(src), "-o", str(ASAN_DIR / (src.stem + ".o"))])
     with cf.ThreadPoolExecutor(max_workers=min(8, os.cpu_count() or 4)) as ex:
         for out in ex.map(_run, jobs):
             if verbose and out.strip():
@@ -72,6 +87,10 @@ def build(verbose: bool = False) -> Path:
     tmp = LIB.with_suffix(".so.tmp")
     _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", str(tmp), *objs, "-lgomp"])
     os.replace(tmp, LIB)
+    # the host-code check with sanitizers: GPU objects as above, host objects
+    # instrumented, one executable (the sanitizer runtimes linked in)
+    san_objs = [str(OBJ_DIR / (s.stem + ".o")) for s in hip + host] + [str(ASAN_DIR / (s.stem + ".o")) for s in san_srcs]
+    _run([HIPCC, f"--offload-arch={ARCH}", *SAN_LINK, "-o", str(HOST_CHECK), *san_objs, "-lgomp"])
     # and TW_Core-style worker drivers, linked against the library in-tree
     exes = [(OUT_DIR / "tw_frame_loop", "frame_loop.cpp"), (OUT_DIR / "tw_icgn_worker", "icgn_worker.cpp"),
             (OUT_DIR / "tw_fftcc_host", "fftcc_host.cpp")]
@@ -79,7 +98,7 @@ def build(verbose: bool = False) -> Path:
         _run([HIPCC, "-O2", "-std=c++17", f"-I{INCLUDE}", str(CPPSRC / "examples" / src),
               "-o", str(exe), f"-L{OUT_DIR}", "-ltwdic", "-Wl,-rpath,$ORIGIN"])
     if verbose:
-        print(f"built {LIB} and {', '.join(str(e) for e, _ in exes)}")
+        print(f"built {LIB}, {HOST_CHECK} and {', '.join(str(e) for e, _ in exes)}")
     return LIB
 
 

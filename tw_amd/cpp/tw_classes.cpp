@@ -75,6 +75,10 @@ void Fftcc2D::setROI(const int_t& iStartX, const int_t& iStartY, const int_t& iR
 // TW_paDIC_FFTCC2D.cpp:88-94
 bool Fftcc2D::recomputeNumPOI()
 {
+    if (m_iGridSpaceX < 1 || m_iGridSpaceY < 1) {   // would divide by zero: no grid
+        m_iNumPOIX = m_iNumPOIY = 0;
+        return false;
+    }
     m_iNumPOIX = int_t(std::floor((m_iROIWidth - m_iSubsetX * 2 - m_iMarginX * 2) / real_t(m_iGridSpaceX))) + 1;
     m_iNumPOIY = int_t(std::floor((m_iROIHeight - m_iSubsetY * 2 - m_iMarginY * 2) / real_t(m_iGridSpaceY))) + 1;
     return m_iNumPOIX > 0 && m_iNumPOIY > 0;

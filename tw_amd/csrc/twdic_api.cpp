@@ -91,11 +91,18 @@ twdic_status check_spans(twdic_ctx* c, const char* call, std::initializer_list<N
     } while (0)
 
 // Fftcc2D::recomputeNumPOI (TW_paDIC_FFTCC2D.cpp:88-94): float division,
-// floor, truncation.
+// floor, truncation (a negative count, as the reference computes it, marks
+// an invalid geometry).  A grid spacing below 1, which the reference would
+// divide by, and a count outside int give 0: no valid grid either.
 void num_poi(const twdic_config& c, int* nx, int* ny)
 {
-    *nx = (int)std::floor((float)(c.roi_w - c.sx * 2 - c.mx * 2) / (float)c.gx) + 1;
-    *ny = (int)std::floor((float)(c.roi_h - c.sy * 2 - c.my * 2) / (float)c.gy) + 1;
+    auto count = [](int roi, int s, int m, int g) {
+        if (g < 1) return 0;
+        const float q = std::floor((float)((int64_t)roi - 2 * (int64_t)s - 2 * (int64_t)m) / (float)g) + 1.f;
+        return (q > -2147483648.f && q < 2147483647.f) ? (int)q : 0;
+    };
+    *nx = count(c.roi_w, c.sx, c.mx, c.gx);
+    *ny = count(c.roi_h, c.sy, c.my, c.gy);
 }
 
 const char* validate(const twdic_config& c)
