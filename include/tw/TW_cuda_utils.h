@@ -4,10 +4,7 @@
 // completes before it returns (the reference launches on the legacy default
 // stream).  int() truncates toward zero, as the reference.
 //
-// Not provided: the 7-argument cuAccumulatePOI overload (accumulatePOINew_Kernel,
-// cuda_utils.cu:203-223), which TW_Core does not call on this path and which
-// subtracts the accumulated U from y and V from x; and the colour-map and
-// min/max helpers (display only).
+// Not provided: the colour-map and min/max helpers (display only).
 #ifndef TWAMD_TW_CUDA_UTILS_H
 #define TWAMD_TW_CUDA_UTILS_H
 
@@ -20,6 +17,13 @@ void cuUpdatePOIpos(TW::real_t* fU, TW::real_t* fV, int iNumberX, int iNumberY, 
 
 // iPOIXY[i] = iCurrentPOIXY[i] + (int(fCurrentV[i]), int(fCurrentU[i])).
 void cuAccumulatePOI(TW::real_t* fCurrentU, TW::real_t* fCurrentV, int* iCurrentPOIXY, int iNumPOI, int* iPOIXY);
+
+// iPOIXY[i] = iCurrentPOIXY[i] + (int(fV[i]) - int(fAccumulateU[i]),
+// int(fU[i]) - int(fAccumulateV[i])): the accumulated U is taken from y and
+// the accumulated V from x, as the reference's accumulatePOINew_Kernel
+// (cuda_utils.cu:203-223) pairs them.
+void cuAccumulatePOI(TW::real_t* fU, TW::real_t* fV, TW::real_t* fAccumulateU, TW::real_t* fAccumulateV,
+                     int* iCurrentPOIXY, int iNumPOI, int* iPOIXY);
 
 // fCurrentU[i] += fU[i], fCurrentV[i] += fV[i] (the first pair accumulates).
 void cuAccumulateUV(TW::real_t* fCurrentU, TW::real_t* fCurrentV, int iPOINum, TW::real_t* fU, TW::real_t* fV);

@@ -51,6 +51,11 @@ public:
                           ICGN2DFlag* flags = nullptr);
     void ICGN2D_Algorithm(real_t* fU, real_t* fV, int* iNumIterations, int* iPOIpos, const MatView& tarImg,
                           float& fPrecomputeTime, float& fICGNTime);
+    // the [head, tail) form of ICGN2D_CPU (TW_paDIC_ICGN2D_CPU.h:71-85): its
+    // implementation (TW_paDIC_ICGN2D_CPU.cpp:198-254) runs every POI and
+    // does not read head or tail, and so does this one
+    void ICGN2D_Algorithm(real_t* fU, real_t* fV, int* iNumIterations, int* iPOIpos, const MatView& tarImg,
+                          int_t head, int_t tail);
 
     // 0 = gather from the materialised LUT (the reference's scheme),
     // 1 = evaluate the same interpolant from the pixels (default)

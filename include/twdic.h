@@ -224,6 +224,13 @@ twdic_status twdic_track_update_poi(twdic_ctx* ctx, int32_t* d_poi_yx, int n, co
  * integer positions in the current target. */
 twdic_status twdic_track_current_poi(twdic_ctx* ctx, const int32_t* d_poi_yx, int n, const float* d_u,
                                      const float* d_v, int32_t* d_out_yx);
+/* Replaces the 7-argument cuAccumulatePOI / accumulatePOINew_Kernel
+ * (cuda_utils.cu:203-223, 305-325): d_out_yx[i] = d_poi_yx[i] +
+ * (int(v[i]) - int(acc_u[i]), int(u[i]) - int(acc_v[i])), the accumulated U
+ * taken from y and the accumulated V from x, as the reference pairs them. */
+twdic_status twdic_track_current_poi_acc(twdic_ctx* ctx, const int32_t* d_poi_yx, int n, const float* d_u,
+                                         const float* d_v, const float* d_acc_u, const float* d_acc_v,
+                                         int32_t* d_out_yx);
 /* Replaces cuAccumulateUV / accumulateUV_Kernel (cuda_utils.cu:225-239,
  * 323-338): d_acc_u[i] += d_u[i], d_acc_v[i] += d_v[i]. */
 twdic_status twdic_track_accumulate_uv(twdic_ctx* ctx, int n, const float* d_u, const float* d_v, float* d_acc_u,

@@ -61,6 +61,7 @@ def lib() -> C.CDLL:
             "orc_track_update_poi": ([vp, i, vp, vp], None),
             "orc_track_current_poi": ([vp, i, vp, vp, vp], None),
             "orc_track_accumulate_uv": ([i, vp, vp, vp, vp], None),
+            "orc_track_current_poi_acc": ([vp, i, vp, vp, vp, vp, vp], None),
             "orc_strain_ls": ([vp, i, i, i, i, vp, vp], None),
         }
         for name, (a, r) in sig.items():
@@ -216,6 +217,15 @@ def track_current_poi(poi_yx, u, v) -> np.ndarray:
     v = np.ascontiguousarray(v, np.float32)
     out = np.empty_like(poi)
     lib().orc_track_current_poi(_p(poi), len(poi), _p(u), _p(v), _p(out))
+    return out
+
+
+def track_current_poi_acc(poi_yx, u, v, acc_u, acc_v) -> np.ndarray:
+    """The 7-argument cuAccumulatePOI (accumulatePOINew_Kernel)."""
+    poi = np.ascontiguousarray(poi_yx, np.int32)
+    u, v, au, av = (np.ascontiguousarray(a, np.float32) for a in (u, v, acc_u, acc_v))
+    out = np.empty_like(poi)
+    lib().orc_track_current_poi_acc(_p(poi), len(poi), _p(u), _p(v), _p(au), _p(av), _p(out))
     return out
 
 

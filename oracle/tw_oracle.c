@@ -158,6 +158,17 @@ void orc_track_current_poi(const int32_t* poi_yx, int n, const float* u, const f
     }
 }
 
+/* cuda_utils.cu:203-223 (accumulatePOINew_Kernel): y takes int(v) - int(accumulated u),
+ * x takes int(u) - int(accumulated v), the pairing the reference writes. */
+void orc_track_current_poi_acc(const int32_t* poi_yx, int n, const float* u, const float* v, const float* acc_u,
+                               const float* acc_v, int32_t* out_yx)
+{
+    for (int i = 0; i < n; ++i) {
+        out_yx[i * 2 + 0] = poi_yx[i * 2 + 0] + (int)v[i] - (int)acc_u[i];
+        out_yx[i * 2 + 1] = poi_yx[i * 2 + 1] + (int)u[i] - (int)acc_v[i];
+    }
+}
+
 /* cuda_utils.cu:225-239 (accumulateUV_Kernel): the first pair accumulates. */
 void orc_track_accumulate_uv(int n, const float* u, const float* v, float* acc_u, float* acc_v)
 {

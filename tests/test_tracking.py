@@ -32,6 +32,18 @@ def test_oracle_tracking_closed_forms():
     assert moved[2].tolist() == [poi[2, 0], poi[2, 1] + 2] and moved[3].tolist() == [poi[3, 0], poi[3, 1] - 2]
     au, av = oracle.track_accumulate_uv(u, v, v, u)
     assert np.array_equal(au, (u + v).astype(np.float32)) and np.array_equal(av, (v + u).astype(np.float32))
+    # the 7-argument form: y takes trunc(v) - trunc(accumulated u), x takes
+    # trunc(u) - trunc(accumulated v); with zero accumulations it is the
+    # 5-argument form, with acc = (u, v) it moves by trunc(v) - trunc(u)
+    acc_u = np.random.default_rng(5).uniform(-9, 9, len(u)).astype(np.float32)
+    acc_v = np.random.default_rng(6).uniform(-9, 9, len(u)).astype(np.float32)
+    got = oracle.track_current_poi_acc(poi, u, v, acc_u, acc_v)
+    assert np.array_equal(got[:, 0], poi[:, 0] + np.trunc(v).astype(np.int32) - np.trunc(acc_u).astype(np.int32))
+    assert np.array_equal(got[:, 1], poi[:, 1] + np.trunc(u).astype(np.int32) - np.trunc(acc_v).astype(np.int32))
+    z = np.zeros_like(u)
+    assert np.array_equal(oracle.track_current_poi_acc(poi, u, v, z, z), moved)
+    same = oracle.track_current_poi_acc(poi, u, v, u, v)
+    assert np.array_equal(same[:, 0] - poi[:, 0], np.trunc(v).astype(np.int32) - np.trunc(u).astype(np.int32))
 
 
 @pytest.mark.gpu
@@ -55,6 +67,15 @@ def test_gpu_tracking_bit_exact():
         assert np.array_equal(dp.download(), want)
         wu, wv = oracle.track_accumulate_uv(v, u, u, v)
         assert np.array_equal(au.download(), wu) and np.array_equal(av.download(), wv)
+        # the 7-argument form (dp has moved: it equals `want` now)
+        acc_u = np.random.default_rng(8).uniform(-30, 30, n).astype(np.float32)
+        acc_v = np.random.default_rng(9).uniform(-30, 30, n).astype(np.float32)
+        dau = ctx.array(n, np.float32).upload(acc_u)
+        dav = ctx.array(n, np.float32).upload(acc_v)
+        out2 = ctx.array(poi.shape, np.int32).fill(0)
+        ctx.track_current_poi_acc(dp, n, du, dv, dau, dav, out2)
+        ctx.sync()
+        assert np.array_equal(out2.download(), oracle.track_current_poi_acc(want, u, v, acc_u, acc_v))
 
 
 @pytest.mark.gpu

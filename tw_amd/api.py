@@ -117,6 +117,7 @@ _SIGS = {
     "twdic_host_free": ([_VP], _I),
     "twdic_track_update_poi": ([_VP, _VP, _I, _VP, _VP], _I),
     "twdic_track_current_poi": ([_VP, _VP, _I, _VP, _VP, _VP], _I),
+    "twdic_track_current_poi_acc": ([_VP, _VP, _I, _VP, _VP, _VP, _VP, _VP], _I),
     "twdic_track_accumulate_uv": ([_VP, _I, _VP, _VP, _VP, _VP], _I),
     "twdic_write_csv": ([C.c_char_p, _I, _VP, _VP, _VP, _VP, _VP, _VP, _VP, _VP], _I),
     "twdic_strain_ls": ([_VP, _VP, _I, _I, _I, _I, _VP, _VP], _I),
@@ -408,6 +409,12 @@ class Context:
     def track_current_poi(self, poi: DeviceArray, n: int, u: DeviceArray, v: DeviceArray, out: DeviceArray):
         _fits(n, poi=poi, u=u, v=v, out=out)
         self._check(lib().twdic_track_current_poi(self.h, poi.ptr, n, u.ptr, v.ptr, out.ptr), "track_current_poi")
+
+    def track_current_poi_acc(self, poi: DeviceArray, n: int, u: DeviceArray, v: DeviceArray, acc_u: DeviceArray,
+                              acc_v: DeviceArray, out: DeviceArray):
+        _fits(n, poi=poi, u=u, v=v, acc_u=acc_u, acc_v=acc_v, out=out)
+        self._check(lib().twdic_track_current_poi_acc(self.h, poi.ptr, n, u.ptr, v.ptr, acc_u.ptr, acc_v.ptr, out.ptr),
+                    "track_current_poi_acc")
 
     def track_accumulate_uv(self, n: int, u: DeviceArray, v: DeviceArray, acc_u: DeviceArray, acc_v: DeviceArray):
         _fits(n, u=u, v=v, acc_u=acc_u, acc_v=acc_v)

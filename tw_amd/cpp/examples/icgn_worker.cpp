@@ -72,6 +72,17 @@ int main(int argc, char** argv)
             icgn.ICGN2D_Algorithm(fU.data(), fV.data(), iters.data(), poi.data(), MatView(tar.data(), H, W, W),
                                   flags.data());
             for (size_t i = 0; i < n; ++i) fl[i] = (unsigned char)flags[i];
+            // the [head, tail) overload runs every POI, as the reference's
+            // does: the same inputs must give the same outputs
+            std::vector<real_t> hU(n), hV(n);
+            std::vector<int> hIt(iters);
+            std::memcpy(hU.data(), uv_raw.data() + 8 * n * k, 4 * n);
+            std::memcpy(hV.data(), uv_raw.data() + 8 * n * k + 4 * n, 4 * n);
+            icgn.ICGN2D_Algorithm(hU.data(), hV.data(), hIt.data(), poi.data(), MatView(tar.data(), H, W, W),
+                                  (int_t)0, (int_t)(n / 2));
+            if (std::memcmp(hU.data(), fU.data(), 4 * n) || std::memcmp(hV.data(), fV.data(), 4 * n) ||
+                hIt != iters)
+                throw std::runtime_error("the head/tail overload differs from the full one");
             std::fwrite(fU.data(), sizeof(real_t), n, fo);
             std::fwrite(fV.data(), sizeof(real_t), n, fo);
             std::fwrite(iters.data(), sizeof(int), n, fo);

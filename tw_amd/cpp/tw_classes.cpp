@@ -446,5 +446,13 @@ void cuICGN2D::ICGN2D_Algorithm(real_t* fU, real_t* fV, int* iNumIterations, int
     fICGNTime = std::chrono::duration<float, std::milli>(t1 - t0).count();
 }
 
+void cuICGN2D::ICGN2D_Algorithm(real_t* fU, real_t* fV, int* iNumIterations, int* iPOIpos, const MatView& tarImg,
+                                int_t head, int_t tail)
+{
+    (void)head;
+    (void)tail;
+    ICGN2D_Algorithm(fU, fV, iNumIterations, iPOIpos, tarImg, nullptr);
+}
+
 }  // namespace paDIC
 }  // namespace TW

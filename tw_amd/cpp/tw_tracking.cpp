@@ -23,6 +23,13 @@ void cuAccumulatePOI(TW::real_t* fCurrentU, TW::real_t* fCurrentV, int* iCurrent
     check(twdic_track_current_poi(nullptr, iCurrentPOIXY, iNumPOI, fCurrentU, fCurrentV, iPOIXY), "cuAccumulatePOI");
 }
 
+void cuAccumulatePOI(TW::real_t* fU, TW::real_t* fV, TW::real_t* fAccumulateU, TW::real_t* fAccumulateV,
+                     int* iCurrentPOIXY, int iNumPOI, int* iPOIXY)
+{
+    check(twdic_track_current_poi_acc(nullptr, iCurrentPOIXY, iNumPOI, fU, fV, fAccumulateU, fAccumulateV, iPOIXY),
+          "cuAccumulatePOI");
+}
+
 void cuAccumulateUV(TW::real_t* fCurrentU, TW::real_t* fCurrentV, int iPOINum, TW::real_t* fU, TW::real_t* fV)
 {
     check(twdic_track_accumulate_uv(nullptr, iPOINum, fU, fV, fCurrentU, fCurrentV), "cuAccumulateUV");

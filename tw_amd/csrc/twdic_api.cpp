@@ -719,6 +719,18 @@ twdic_status twdic_track_current_poi(twdic_ctx* c, const int32_t* d_poi, int n, 
     return track_finish(c, launch_track_current_poi(d_poi, n, d_u, d_v, d_out, c ? c->stream : nullptr));
 }
 
+twdic_status twdic_track_current_poi_acc(twdic_ctx* c, const int32_t* d_poi, int n, const float* d_u,
+                                         const float* d_v, const float* d_acc_u, const float* d_acc_v, int32_t* d_out)
+{
+    if (n < 0 || (n > 0 && (!d_poi || !d_u || !d_v || !d_acc_u || !d_acc_v || !d_out)))
+        return c ? fail(c, TWDIC_EINVAL, "track_current_poi_acc: bad arguments") : TWDIC_EINVAL;
+    TW_SPANS(c, "track_current_poi_acc", {d_poi, 8 * (size_t)n, "d_poi"}, {d_u, 4 * (size_t)n, "d_u"},
+             {d_v, 4 * (size_t)n, "d_v"}, {d_acc_u, 4 * (size_t)n, "d_acc_u"}, {d_acc_v, 4 * (size_t)n, "d_acc_v"},
+             {d_out, 8 * (size_t)n, "d_out"});
+    return track_finish(c, launch_track_current_poi_acc(d_poi, n, d_u, d_v, d_acc_u, d_acc_v, d_out,
+                                                        c ? c->stream : nullptr));
+}
+
 twdic_status twdic_track_accumulate_uv(twdic_ctx* c, int n, const float* d_u, const float* d_v, float* d_acc_u,
                                        float* d_acc_v)
 {

@@ -148,6 +148,8 @@ hipError_t launch_strain_ls(const float* f, int nx, int ny, int grid, int M, dou
 hipError_t launch_track_update_poi(int32_t* poi, int n, const float* u, const float* v, hipStream_t st);
 hipError_t launch_track_current_poi(const int32_t* poi, int n, const float* u, const float* v, int32_t* out,
                                     hipStream_t st);
+hipError_t launch_track_current_poi_acc(const int32_t* poi, int n, const float* u, const float* v, const float* acc_u,
+                                        const float* acc_v, int32_t* out, hipStream_t st);
 hipError_t launch_track_accumulate_uv(int n, const float* u, const float* v, float* acc_u, float* acc_v,
                                       hipStream_t st);
 // occupancy-derived persistent grid sizes
