@@ -31,7 +31,9 @@ typedef struct {
     int slots;              /* requests in flight at once, one HIP stream each (1..64) */
     int queue_capacity;     /* admission queue length (Concurrent_Buffer size) */
     size_t hbm_budget;      /* device bytes for references, resident state and in-flight
-                               requests; 0 = a quarter of the free device memory */
+                               requests (a w x h request holds its frame, 16 (w - 2) (h - 1)
+                               bytes of row polynomials and 41 bytes per POI);
+                               0 = a quarter of the free device memory */
     int max_width;          /* largest frame accepted */
     int max_height;
     int max_pois;           /* largest POI set accepted */

@@ -130,8 +130,8 @@ def test_reference_replacement_in_flight(world):
 def test_memory_admission_and_dropping(world):
     # budget: one reference (1 MiB image + 4 MiB gradients), the POI set and
     # state (20,000 x 392 B) and about two requests' buffers (1 MiB frame +
-    # 0.8 MiB outputs each)
-    budget = 16 << 20
+    # 16 MiB of its row polynomials + 0.8 MiB outputs each)
+    budget = 50 << 20
     with api.Server(slots=8, queue_capacity=8, hbm_budget=budget, max_width=W, max_height=H,
                     max_pois=20000) as srv:
         srv.set_reference(0, world["refs"][0])

@@ -39,6 +39,8 @@ struct twdic_batch {
     int64_t nf[3] = {};
     int64_t total = 0;
     uint32_t* d_g2 = nullptr;         // reference gradient planes of all pairs
+    uint4* d_rp = nullptr;            // target row polynomials of all pairs (rebuilt every step)
+    int max_rh = 0;                   // largest ROI height among the pairs with a plane
     double* d_state = nullptr;        // per-POI reference state of all pairs
     float* d_zncc_scratch = nullptr;  // for pairs without a ZNCC output
     std::map<int, float2*> tw;        // FFT twiddle tables per window edge
@@ -105,7 +107,7 @@ void release(twdic_batch* b)
     b->graph = nullptr;
     void* bufs[] = {b->d_desc, b->d_items, b->d_fitems[0], b->d_fitems[1], b->d_fitems[2], b->d_g2, b->d_state,
                     b->d_zncc_scratch, b->d_counters, b->d_mitems[0], b->d_mitems[1], b->d_mitems[2],
-                    b->d_mitems[3], b->d_wg[0], b->d_wg[1]};
+                    b->d_mitems[3], b->d_wg[0], b->d_wg[1], b->d_rp};
     for (void* p : bufs)
         if (p) (void)hipFree(p);
     for (int f = 0; f < 4; ++f) { b->d_mitems[f] = nullptr; b->nm[f] = 0; }
@@ -116,6 +118,7 @@ void release(twdic_batch* b)
     b->d_desc = nullptr; b->d_items = nullptr;
     b->d_fitems[0] = b->d_fitems[1] = b->d_fitems[2] = nullptr;
     b->d_g2 = nullptr; b->d_state = nullptr; b->d_zncc_scratch = nullptr; b->d_counters = nullptr;
+    b->d_rp = nullptr; b->max_rh = 0;
     b->bytes = 0;
 }
 
@@ -207,6 +210,7 @@ twdic_status enqueue_frame(twdic_batch* b, bool fftcc, bool icgn)
                                          b->pblocks_fftcc[c][b->tune.fftcc_variant], b->stream));
         }
     }
+    if (icgn) TB_HIP(b, launch_batch_row_poly(b->d_desc, (int)b->pairs.size(), b->max_rh, b->stream));
     if (icgn)
         TB_HIP(b, launch_batch_icgn(b->d_desc, b->d_items, (int)b->total, b->d_counters + CNT_ICGN,
                                     std::max(1, b->tune.icgn_chunk), b->pblocks_icgn, b->stream));
@@ -232,6 +236,8 @@ twdic_status run_direct(twdic_batch* b, bool fftcc, bool icgn)
         twdic_status st = build_state_direct(b);
         if (st != TWDIC_OK) return st;
     }
+    // every pair's row polynomials of this step's targets, before the fork
+    if (icgn) TB_HIP(b, launch_batch_row_poly(b->d_desc, (int)b->pairs.size(), b->max_rh, b->stream));
     // fork: every lane stream waits for the work already on the batch stream
     const int L = std::max(1, std::min(b->lanes, twdic_batch::kLanes));
     TB_HIP(b, hipEventRecord(b->fork, b->stream));
@@ -259,6 +265,7 @@ twdic_status enqueue_flat(twdic_batch* b, bool fftcc, bool icgn)
     if (fftcc)
         for (int f = 0; f < 4; ++f)
             if (b->nm[f]) TB_HIP(b, launch_batch_fftcc_mma(b->d_desc, b->d_mitems[f], (int)b->nm[f], f, b->stream));
+    if (icgn) TB_HIP(b, launch_batch_row_poly(b->d_desc, (int)b->pairs.size(), b->max_rh, b->stream));
     if (icgn)
         for (int t = 0; t < 2; ++t)
             if (b->nwg[t])
@@ -321,10 +328,11 @@ twdic_status set_pairs_impl(twdic_batch* b, const twdic_pair* pairs, int npairs)
     release(b);
     b->pairs.assign(pairs, pairs + npairs);
     b->h_desc.assign(npairs, PairDesc{});
-    size_t g2_elems = 0, state_elems = 0, zn = 0;
+    size_t g2_elems = 0, state_elems = 0, zn = 0, rp_elems = 0;
     int64_t total = 0;
     for (const auto& p : b->pairs) {
         g2_elems += (size_t)p.roi_w * p.roi_h;
+        if (row_poly_fits(p.roi_x, p.roi_y, p.roi_w, p.roi_h)) rp_elems += row_poly_bytes(p.roi_w, p.roi_h) / 16;
         state_elems += (size_t)p.n * kRefStateStride;
         if (!p.d_zncc) zn += p.n;
         total += p.n;
@@ -336,6 +344,7 @@ twdic_status set_pairs_impl(twdic_batch* b, const twdic_pair* pairs, int npairs)
         return e;
     };
     TB_HIP(b, alloc((void**)&b->d_g2, g2_elems * sizeof(uint32_t)));
+    TB_HIP(b, alloc((void**)&b->d_rp, rp_elems * sizeof(uint4)));
     TB_HIP(b, alloc((void**)&b->d_state, state_elems * sizeof(double)));
     TB_HIP(b, alloc((void**)&b->d_zncc_scratch, zn * sizeof(float)));
     TB_HIP(b, alloc((void**)&b->d_desc, sizeof(PairDesc) * std::max(1, npairs)));
@@ -344,7 +353,7 @@ twdic_status set_pairs_impl(twdic_batch* b, const twdic_pair* pairs, int npairs)
     std::vector<int2> items;
     std::vector<int2> fitems[3];
     items.reserve(total);
-    size_t g2_off = 0, st_off = 0, zn_off = 0;
+    size_t g2_off = 0, st_off = 0, zn_off = 0, rp_off = 0;
     for (int i = 0; i < npairs; ++i) {
         const twdic_pair& p = b->pairs[i];
         if (!b->tw.count(p.fft_n)) {
@@ -364,6 +373,14 @@ twdic_status set_pairs_impl(twdic_batch* b, const twdic_pair* pairs, int npairs)
         q.Sx = p.sx; q.Sy = p.sy; q.Wsub = 2 * p.sx + 1; q.Hsub = 2 * p.sy + 1; q.s = q.Wsub * q.Hsub;
         q.max_iter = p.max_iter; q.delta_p = p.delta_p;
         q.g2 = b->d_g2 + g2_off; q.lut = nullptr;
+        // the fused interpolation's row polynomials, as the single context
+        // builds them (ROIs past 2^28 pixels read the image bytes there too)
+        q.rp = nullptr;
+        if (row_poly_fits(p.roi_x, p.roi_y, p.roi_w, p.roi_h)) {
+            q.rp = b->d_rp + rp_off;
+            rp_off += row_poly_bytes(p.roi_w, p.roi_h) / 16;
+            b->max_rh = std::max(b->max_rh, p.roi_h);
+        }
         q.refstate = b->d_state + st_off;
         q.poi = p.d_poi_yx; q.n = p.n;
         q.u = p.d_u; q.v = p.d_v; q.p6 = p.d_p6; q.iters = p.d_iters; q.flags = p.d_flags; q.dpnorm = nullptr;

@@ -66,12 +66,57 @@ __device__ __forceinline__ void catmull_rom_w(float t, float w[4])
     w[3] = t2 * fmaf(0.5f, t, -0.5f);
 }
 
-__device__ __forceinline__ float row_dot(uint32_t w, const float wx[4])
+// One neighbourhood row (its four pixels are the bytes of r) against the x
+// weights.  The bytes are widened into 16-bit halves, where each is read as
+// the fp16 subnormal byte * 2^-24, exactly; the mixed-precision FMA takes such
+// a half as an exact f32 operand, so no conversion instruction is spent per
+// pixel.  The result is 2^-24 times the weighted sum taken in the same order
+// with the same roundings (a power-of-two scale commutes with rounding while
+// nothing reaches the f32 subnormal range, and every product here is above
+// 2^-80).  The first term adds +0 where a plain product would give -0 for a
+// negative weight on a zero pixel; the row sums are equal otherwise.
+typedef _Float16 half2_t __attribute__((ext_vector_type(2)));
+
+__device__ __forceinline__ half2_t bytes02(uint32_t r)
 {
-    float s = wx[0] * (float)(w & 255u);
-    s = fmaf(wx[1], (float)((w >> 8) & 255u), s);
-    s = fmaf(wx[2], (float)((w >> 16) & 255u), s);
-    return fmaf(wx[3], (float)(w >> 24), s);
+    return __builtin_bit_cast(half2_t, r & 0x00ff00ffu);
+}
+
+__device__ __forceinline__ half2_t bytes13(uint32_t r)
+{
+    return __builtin_bit_cast(half2_t, __builtin_amdgcn_perm(0u, r, 0x0c030c01u));
+}
+
+__device__ __forceinline__ float row_dot_s(uint32_t r, const float wx[4])
+{
+    const half2_t e = bytes02(r), o = bytes13(r);
+    float s = fmaf(wx[0], (float)e.x, 0.f);
+    s = fmaf(wx[1], (float)o.x, s);
+    s = fmaf(wx[2], (float)e.y, s);
+    return fmaf(wx[3], (float)o.y, s);
+}
+
+// Bicubic value of a 4x4 neighbourhood given as four row words, scaled by
+// 2^-24 (see row_dot_s).  A uniform neighbourhood returns its value exactly
+// (as the LUT does: all higher coefficients vanish), so uniform target regions
+// keep the reference's DarkSubset behaviour: rows 0, 2, 3 equal row 1, and
+// row 1 equals itself turned by a byte, i.e. a chain of byte
+// absolute-difference sums is zero.
+__device__ __forceinline__ float interp_rows_s(uint32_t r0, uint32_t r1, uint32_t r2, uint32_t r3, float tx,
+                                               float ty)
+{
+    float wx[4], wy[4];
+    catmull_rom_w(tx, wx);
+    catmull_rom_w(ty, wy);
+    float acc = wy[0] * row_dot_s(r0, wx);
+    acc = fmaf(wy[1], row_dot_s(r1, wx), acc);
+    acc = fmaf(wy[2], row_dot_s(r2, wx), acc);
+    acc = fmaf(wy[3], row_dot_s(r3, wx), acc);
+    uint32_t diff = __builtin_amdgcn_sad_u8(r0, r1, 0u);
+    diff = __builtin_amdgcn_sad_u8(r2, r1, diff);
+    diff = __builtin_amdgcn_sad_u8(r3, r1, diff);
+    diff = __builtin_amdgcn_sad_u8(r1, __builtin_amdgcn_alignbit(r1, r1, 8), diff);
+    return diff == 0u ? (float)bytes13(r1).x : acc;
 }
 
 // The target image as a buffer resource whose base is the pixel (-1, -1):
@@ -100,30 +145,64 @@ __device__ __forceinline__ int target_offset(const TargetBuf& B, int ix, int iy)
     return (int)__umul24((uint32_t)iy, (uint32_t)B.stride) + ix;
 }
 
-// One unaligned dword per neighbourhood row.  A uniform neighbourhood returns
-// its value exactly (as the LUT does: all higher coefficients vanish), so
-// uniform target regions keep the reference's DarkSubset behaviour.
-__device__ __forceinline__ float interp_fused(const TargetBuf& B, int ix, int iy, float tx, float ty)
+// One unaligned dword per neighbourhood row; the value is scaled by 2^-24.
+__device__ __forceinline__ float interp_fused_s(const TargetBuf& B, int ix, int iy, float tx, float ty)
 {
     const int o = target_offset(B, ix, iy);
     const uint32_t r0 = __builtin_amdgcn_raw_buffer_load_b32(B.r, o, 0, 0);
     const uint32_t r1 = __builtin_amdgcn_raw_buffer_load_b32(B.r, o, B.stride, 0);
     const uint32_t r2 = __builtin_amdgcn_raw_buffer_load_b32(B.r, o, 2 * B.stride, 0);
     const uint32_t r3 = __builtin_amdgcn_raw_buffer_load_b32(B.r, o, 3 * B.stride, 0);
-    float wx[4], wy[4];
-    catmull_rom_w(tx, wx);
-    catmull_rom_w(ty, wy);
-    float acc = wy[0] * row_dot(r0, wx);
-    acc = fmaf(wy[1], row_dot(r1, wx), acc);
-    acc = fmaf(wy[2], row_dot(r2, wx), acc);
-    acc = fmaf(wy[3], row_dot(r3, wx), acc);
-    // uniform: rows 0, 2, 3 equal row 1, and row 1 equals itself turned by a
-    // byte: a chain of byte absolute-difference sums is zero
-    uint32_t diff = __builtin_amdgcn_sad_u8(r0, r1, 0u);
-    diff = __builtin_amdgcn_sad_u8(r2, r1, diff);
-    diff = __builtin_amdgcn_sad_u8(r3, r1, diff);
-    diff = __builtin_amdgcn_sad_u8(r1, __builtin_amdgcn_alignbit(r1, r1, 8), diff);
-    return diff == 0u ? (float)((r1 >> 8) & 255u) : acc;
+    return interp_rows_s(r0, r1, r2, r3, tx, ty);
+}
+
+// The target's row polynomials (launch_row_poly) as a buffer resource whose
+// base is entry -(sy0 rw + sx0): cell (ix, iy) reads the 16-byte entries at
+// byte offsets 16 (iy rw + ix) (rows iy - 1, iy) and that plus 32 rw (rows
+// iy + 1, iy + 2).  The host passes the plane only when every such offset is
+// below 2^32.
+struct RowPolyBuf {
+    __amdgpu_buffer_rsrc_t r;
+    int rw16;      // 16 rw: bytes per entry row
+};
+
+__device__ __forceinline__ RowPolyBuf row_poly_buf(const IcgnParams& P)
+{
+    const uint32_t lead = 16u * ((uint32_t)P.sy0 * (uint32_t)P.rw + (uint32_t)P.sx0);
+    const uint8_t* base = reinterpret_cast<const uint8_t*>(P.rp) - lead;
+    const int records = (int)(lead + 16u * (uint32_t)(P.rh + 1) * (uint32_t)P.rw);
+    return {__builtin_amdgcn_make_buffer_rsrc((void*)base, (short)0, records, 0x00020000), 16 * P.rw};
+}
+
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+
+// One row's cubic at t: c0 + (t/2) (c1 + t (c2 + t c3)), the fp16
+// coefficients taken as exact f32 operands by the mixed-precision FMA.
+__device__ __forceinline__ float row_poly(uint32_t c01, uint32_t c23, float t, float ht)
+{
+    const half2_t a = __builtin_bit_cast(half2_t, c01), b = __builtin_bit_cast(half2_t, c23);
+    const float r = fmaf(fmaf(t, (float)b.y, (float)b.x), t, (float)a.y);
+    return fmaf(ht, r, (float)a.x);
+}
+
+// Bicubic value at (ix + tx, iy + ty): the four rows' cubics at tx, then the
+// Catmull-Rom cubic through them at ty in the same form.  Its coefficients
+// are formed from the row values with exact small-integer weights, so a
+// uniform neighbourhood gives back its value exactly (every higher
+// coefficient is an exact zero), as the LUT does: uniform target regions keep
+// the reference's DarkSubset behaviour without a test.
+__device__ __forceinline__ float interp_row_poly(const RowPolyBuf& B, int ix, int iy, float tx, float ty)
+{
+    const int o = (int)__umul24((uint32_t)iy, (uint32_t)B.rw16) + 16 * ix;
+    const u32x4 a = __builtin_amdgcn_raw_buffer_load_b128(B.r, o, 0, 0);
+    const u32x4 b = __builtin_amdgcn_raw_buffer_load_b128(B.r, o, 2 * B.rw16, 0);
+    const float hx = 0.5f * tx;
+    const float R0 = row_poly(a.x, a.y, tx, hx), R1 = row_poly(a.z, a.w, tx, hx);
+    const float R2 = row_poly(b.x, b.y, tx, hx), R3 = row_poly(b.z, b.w, tx, hx);
+    const float d1 = R2 - R0;
+    const float d2 = fmaf(4.f, R2, fmaf(-5.f, R1, fmaf(2.f, R0, -R3)));
+    const float d3 = fmaf(3.f, R1 - R2, R3 - R0);
+    return fmaf(0.5f * ty, fmaf(fmaf(d3, ty, d2), ty, d1), R1);
 }
 
 // W(p) o W(dp)^-1 and p from W: TW_paDIC_ICGN2D_CPU.cpp:713-745 / :875-907,
@@ -194,16 +273,21 @@ __device__ __forceinline__ void fill_step_table(StepTable& t, const IcgnParams& 
 // table (tab != null) the lanes take pixels lane + 64 k from it: s / 64 full
 // rounds, then the s % 64 remaining pixels; without one they step their
 // offsets in registers.  Each lane visits the same pixels in the same order
-// either way, so the sums are identical.
-template <int INTERP, bool INT, bool CHECK, bool TABLE>
+// either way, so the sums are identical.  RP: the fused bicubic passes
+// evaluate the target's row polynomials (P.rp) instead of the image bytes.
+template <int INTERP, bool INT, bool CHECK, bool TABLE, bool RP>
 __device__ __forceinline__ bool subset_loop(const IcgnParams& P, int px, int py, const float Wm[6], int lane,
                                             double acc[8], const StepTable* tab)
 {
+    // the fused interpolation from the bytes returns T 2^-24 (interp_rows_s);
+    // the sums are scaled back at the end, exactly
+    constexpr bool SCALED = !INT && INTERP == TWDIC_INTERP_FUSED && !RP;
     const int Sx = P.Sx, Sy = P.Sy, Wsub = P.Wsub, s = P.s;
     const int sx0 = P.sx0, sy0 = P.sy0, rw = P.rw, rh = P.rh;
     const TargetBuf tb = target_buf(P);
     const __amdgpu_buffer_rsrc_t g2r =
         __builtin_amdgcn_make_buffer_rsrc((void*)P.g2, (short)0, 4 * rw * rh, 0x00020000);
+    const RowPolyBuf rb = RP ? row_poly_buf(P) : RowPolyBuf{};
     const float fpx = (float)px, fpy = (float)py;
     double st = 0.0, stt = 0.0, a0 = 0.0, a1 = 0.0, a2 = 0.0, b0 = 0.0, b1 = 0.0, b2 = 0.0;
     bool out = false;
@@ -224,10 +308,13 @@ __device__ __forceinline__ bool subset_loop(const IcgnParams& P, int px, int py,
         } else if (interior) {
             // :771-772; wx, wy >= 1 here, so x - (float)(int)x is fract(x), exactly
             const float tx = __builtin_amdgcn_fractf(wx), ty = __builtin_amdgcn_fractf(wy);
-            if (INTERP == TWDIC_INTERP_FUSED)
-                T = interp_fused(tb, ix, iy, tx, ty);
-            else
+            if constexpr (RP) {
+                T = interp_row_poly(rb, ix, iy, tx, ty);
+            } else if (INTERP == TWDIC_INTERP_FUSED) {
+                T = interp_fused_s(tb, ix, iy, tx, ty);
+            } else {
                 T = interp_lut(P.lut, (uint32_t)((iy - sy0) * rw + (ix - sx0)), tx, ty);
+            }
         }
         const uint32_t g = __builtin_amdgcn_raw_buffer_load_b32(g2r, (int)go, goff, 0);
         // FP64: every product SD_k T is exact, and where T takes integer values
@@ -286,9 +373,12 @@ __device__ __forceinline__ bool subset_loop(const IcgnParams& P, int px, int py,
             go += wrap ? gstep + gwrap : gstep;
         }
     }
-    acc[0] = st; acc[1] = stt;
-    acc[2] = 0.5 * a0; acc[3] = 0.5 * a1; acc[4] = 0.5 * a2;
-    acc[5] = 0.5 * b0; acc[6] = 0.5 * b1; acc[7] = 0.5 * b2;
+    // 2^24 and 2^48 scale the sums of T 2^-24 back exactly (FP64, far from
+    // its range limits)
+    const double sc = SCALED ? 0x1p24 : 1.0, sc2 = SCALED ? 0x1p48 : 1.0, hsc = SCALED ? 0x1p23 : 0.5;
+    acc[0] = sc * st; acc[1] = sc2 * stt;
+    acc[2] = hsc * a0; acc[3] = hsc * a1; acc[4] = hsc * a2;
+    acc[5] = hsc * b0; acc[6] = hsc * b1; acc[7] = hsc * b2;
     return CHECK ? !__any(out) : true;
 }
 
@@ -301,7 +391,7 @@ __device__ __forceinline__ bool subset_loop(const IcgnParams& P, int px, int py,
 // region where no per-pixel test can fire (the margin covers FP32 rounding
 // of the per-pixel warp, which is below 1e-3 px), the pass runs without the
 // per-pixel tests; the results are identical either way.
-template <int INTERP, bool INT, bool TABLE>
+template <int INTERP, bool INT, bool TABLE, bool RP>
 __device__ __forceinline__ bool subset_pass(const IcgnParams& P, int px, int py, const float Wm[6], int lane,
                                             double acc[8], const StepTable* tab)
 {
@@ -314,14 +404,14 @@ __device__ __forceinline__ bool subset_pass(const IcgnParams& P, int px, int py,
                       fmaxf(fmaxf(x0, x1), fmaxf(x2, x3)) < (float)(P.sx0 + P.rw - 2) &&
                       fminf(fminf(y0, y1), fminf(y2, y3)) >= (float)(P.sy0 + 1) &&
                       fmaxf(fmaxf(y0, y1), fmaxf(y2, y3)) < (float)(P.sy0 + P.rh - 2);
-    if (safe) return subset_loop<INTERP, INT, false, TABLE>(P, px, py, Wm, lane, acc, tab);
-    return subset_loop<INTERP, INT, true, TABLE>(P, px, py, Wm, lane, acc, tab);
+    if (safe) return subset_loop<INTERP, INT, false, TABLE, RP>(P, px, py, Wm, lane, acc, tab);
+    return subset_loop<INTERP, INT, true, TABLE, RP>(P, px, py, Wm, lane, acc, tab);
 }
 
 // IC-GN for one POI by one wavefront.  poi is wave-uniform, so per-POI state
 // is read with scalar loads into SGPRs.
-template <int INTERP, bool TABLE>
-__device__ __forceinline__ void icgn_poi(const IcgnParams& P, int poi, int lane, const StepTable* tab)
+template <int INTERP, bool TABLE, bool RP>
+__device__ __forceinline__ void icgn_poi_t(const IcgnParams& P, int poi, int lane, const StepTable* tab)
 {
     const int py = __builtin_amdgcn_readfirstlane(P.poi[2 * poi]);
     const int px = __builtin_amdgcn_readfirstlane(P.poi[2 * poi + 1]);
@@ -344,8 +434,8 @@ __device__ __forceinline__ void icgn_poi(const IcgnParams& P, int poi, int lane,
         for (int it = 0;; ++it) {
             if (it >= 1) iters = it + 1;                       // ++iNumIterations (:753)
             double acc[8];
-            const bool inside = it == 0 ? subset_pass<INTERP, true, TABLE>(P, px, py, Wm, lane, acc, tab)
-                                        : subset_pass<INTERP, false, TABLE>(P, px, py, Wm, lane, acc, tab);
+            const bool inside = it == 0 ? subset_pass<INTERP, true, TABLE, RP>(P, px, py, Wm, lane, acc, tab)
+                                        : subset_pass<INTERP, false, TABLE, RP>(P, px, py, Wm, lane, acc, tab);
             if (!inside) { flag = TWDIC_ICGN_OUT_OF_ROI; break; }         // :632-635, :787-790
 #pragma unroll
             for (int i = 0; i < 8; ++i) acc[i] = wave_sum(acc[i]);
@@ -411,6 +501,21 @@ __device__ __forceinline__ void icgn_poi(const IcgnParams& P, int poi, int lane,
     }
 }
 
+// The fused mode reads the target's row polynomials when the caller built
+// them (P.rp, the same for every wave of a launch), else the image bytes;
+// both evaluate the same Catmull-Rom spline.
+template <int INTERP, bool TABLE>
+__device__ __forceinline__ void icgn_poi(const IcgnParams& P, int poi, int lane, const StepTable* tab)
+{
+    if (INTERP == TWDIC_INTERP_FUSED && P.rp != nullptr)
+        icgn_poi_t<INTERP, TABLE, INTERP == TWDIC_INTERP_FUSED>(P, poi, lane, tab);
+    else
+        icgn_poi_t<INTERP, TABLE, false>(P, poi, lane, tab);
+}
+
+// The single-context kernel takes the choice at launch instead (RP: P.rp is
+// set), so each form gets its own register budget.
+
 // MINW: occupancy target (waves per SIMD) the register budget is fitted to;
 // 0 leaves the choice to the compiler.
 // TABLE: the subset has at most kStepTable pixels; the workgroup builds the
@@ -424,7 +529,7 @@ __device__ __forceinline__ int xcd_block(int b, int nb)
     return x * q + min(x, r) + slot;
 }
 
-template <int INTERP, int MINW, bool TABLE>
+template <int INTERP, int MINW, bool TABLE, bool RP>
 __global__ __launch_bounds__(256, MINW > 0 ? MINW : 1) void k_icgn(IcgnParams P)
 {
     const int poi = __builtin_amdgcn_readfirstlane(xcd_block(blockIdx.x, gridDim.x) * 4 + (threadIdx.x >> 6));
@@ -433,10 +538,10 @@ __global__ __launch_bounds__(256, MINW > 0 ? MINW : 1) void k_icgn(IcgnParams P)
         fill_step_table(tab, P);
         __syncthreads();
         if (poi >= P.n) return;
-        icgn_poi<INTERP, true>(P, poi, threadIdx.x & 63, &tab);
+        icgn_poi_t<INTERP, true, RP>(P, poi, threadIdx.x & 63, &tab);
     } else {
         if (poi >= P.n) return;
-        icgn_poi<INTERP, false>(P, poi, threadIdx.x & 63, nullptr);
+        icgn_poi_t<INTERP, false, RP>(P, poi, threadIdx.x & 63, nullptr);
     }
 }
 
@@ -534,25 +639,34 @@ hipError_t launch_batch_icgn(const PairDesc* pairs, const int2* items, int total
     return hipGetLastError();
 }
 
-template <int INTERP, bool TABLE>
+template <int INTERP, bool TABLE, bool RP>
 static void launch_icgn_v(const IcgnParams& p, int variant, hipStream_t st)
 {
     const dim3 grid((p.n + 3) / 4);
     switch (variant) {
-    case 5: hipLaunchKernelGGL((k_icgn<INTERP, 5, TABLE>), grid, dim3(256), 0, st, p); break;
-    case 6: hipLaunchKernelGGL((k_icgn<INTERP, 6, TABLE>), grid, dim3(256), 0, st, p); break;
-    case 7: hipLaunchKernelGGL((k_icgn<INTERP, 7, TABLE>), grid, dim3(256), 0, st, p); break;
-    default: hipLaunchKernelGGL((k_icgn<INTERP, 0, TABLE>), grid, dim3(256), 0, st, p); break;
+    case 5: hipLaunchKernelGGL((k_icgn<INTERP, 5, TABLE, RP>), grid, dim3(256), 0, st, p); break;
+    case 6: hipLaunchKernelGGL((k_icgn<INTERP, 6, TABLE, RP>), grid, dim3(256), 0, st, p); break;
+    case 7: hipLaunchKernelGGL((k_icgn<INTERP, 7, TABLE, RP>), grid, dim3(256), 0, st, p); break;
+    default: hipLaunchKernelGGL((k_icgn<INTERP, 0, TABLE, RP>), grid, dim3(256), 0, st, p); break;
     }
+}
+
+template <int INTERP, bool RP>
+static void launch_icgn_r(const IcgnParams& p, int variant, hipStream_t st)
+{
+    if (p.s <= kStepTable)
+        launch_icgn_v<INTERP, true, RP>(p, variant, st);
+    else
+        launch_icgn_v<INTERP, false, RP>(p, variant, st);
 }
 
 template <int INTERP>
 static void launch_icgn_t(const IcgnParams& p, int variant, hipStream_t st)
 {
-    if (p.s <= kStepTable)
-        launch_icgn_v<INTERP, true>(p, variant, st);
+    if (INTERP == TWDIC_INTERP_FUSED && p.rp != nullptr)
+        launch_icgn_r<INTERP, INTERP == TWDIC_INTERP_FUSED>(p, variant, st);
     else
-        launch_icgn_v<INTERP, false>(p, variant, st);
+        launch_icgn_r<INTERP, false>(p, variant, st);
 }
 
 hipError_t launch_icgn(const IcgnParams& p, int interp, int variant, hipStream_t st)

@@ -6,6 +6,7 @@
 // are bit-identical to the CPU path in any summation order.
 #include "twdic_internal.h"
 
+#include <algorithm>
 #include <cfloat>
 
 namespace twdic {
@@ -213,6 +214,86 @@ hipError_t launch_bicubic_lut(const uint8_t* tar, int stride, int sx0, int sy0, 
 {
     dim3 grid((rw + 63) / 64, (rh + 3) / 4);
     hipLaunchKernelGGL(k_bicubic_lut, grid, dim3(256), 0, st, tar, stride, sx0, sy0, rw, rh, lut);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Row polynomials of the target for the fused bicubic interpolation.  The
+// bicubic LUT above, built from central-difference corner derivatives, is the
+// tensor-product Catmull-Rom spline; along one image row y, between pixels x
+// and x + 1, its cubic through p0..p3 = T(y, x-1 .. x+2) is
+//   R(t) = c0 + t/2 (c1 + t (c2 + t c3)),
+//   c0 = p1, c1 = p2 - p0, c2 = 2 p0 - 5 p1 + 4 p2 - p3, c3 = 3 (p1 - p2) + p3 - p0,
+// integers of magnitude at most 1530, so exact in fp16.  Entry (x, j) holds
+// the four coefficients of rows y = sy0 - 1 + j and y + 1 (fp16, c0..c3 of
+// row y, then of row y + 1: 16 bytes) for cell column sx0 + x, x < rw,
+// j <= rh; the last column (never an interior cell) is zero.  A cell's four
+// neighbourhood rows are then two 16-byte entries, (x, j) and (x, j + 2).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint32_t h2bits(int lo, int hi)
+{
+    const uint32_t a = __builtin_bit_cast(uint16_t, (_Float16)(float)lo);
+    const uint32_t b = __builtin_bit_cast(uint16_t, (_Float16)(float)hi);
+    return a | (b << 16);
+}
+
+// Entry rows j0, j0 + jstep, ... (four per 256-thread block, one per
+// wavefront) of one ROI's plane.
+__device__ __forceinline__ void row_poly_rows(const uint8_t* __restrict__ tar, int stride, int sx0, int sy0, int rw,
+                                              int rh, uint4* __restrict__ rp, int j0, int jstep)
+{
+    for (int j = j0 + (int)(threadIdx.x >> 6); j <= rh; j += jstep) {
+        const uint8_t* b = tar + (size_t)(sy0 - 1 + j) * stride + (sx0 - 1);
+        for (int x = threadIdx.x & 63; x < rw; x += 64) {
+            uint4 e = make_uint4(0u, 0u, 0u, 0u);
+            if (x < rw - 1) {
+                // rows sy0 - 1 + j, +1 and columns sx0 + x - 1 .. sx0 + x + 2
+                // lie in the image: the ROI keeps a one-pixel frame
+                uint32_t h[4];
+#pragma unroll
+                for (int r = 0; r < 2; ++r) {
+                    const uint8_t* q = b + (size_t)r * stride + x;
+                    const int p0 = q[0], p1 = q[1], p2 = q[2], p3 = q[3];
+                    h[2 * r] = h2bits(p1, p2 - p0);
+                    h[2 * r + 1] = h2bits(2 * p0 - 5 * p1 + 4 * p2 - p3, 3 * (p1 - p2) + p3 - p0);
+                }
+                e = make_uint4(h[0], h[1], h[2], h[3]);
+            }
+            rp[(size_t)j * rw + x] = e;
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void k_row_poly(const uint8_t* __restrict__ tar, int stride, int sx0, int sy0,
+                                                  int rw, int rh, uint4* __restrict__ rp)
+{
+    row_poly_rows(tar, stride, sx0, sy0, rw, rh, rp, 4 * blockIdx.x, 4 * gridDim.x);
+}
+
+hipError_t launch_row_poly(const uint8_t* tar, int stride, int sx0, int sy0, int rw, int rh, uint4* rp,
+                           hipStream_t st)
+{
+    const int blocks = std::min((rh + 1 + 3) / 4, 4096);
+    hipLaunchKernelGGL(k_row_poly, dim3(blocks), dim3(256), 0, st, tar, stride, sx0, sy0, rw, rh, rp);
+    return hipGetLastError();
+}
+
+// Batched form: blockIdx.y is the pair; geometry and buffers come from its
+// descriptor (read when the kernel runs, so a captured graph follows new
+// targets).  Pairs without a plane (rp null) are skipped.
+__global__ __launch_bounds__(256) void k_batch_row_poly(const PairDesc* __restrict__ pairs)
+{
+    const IcgnParams P = const_pairs(pairs)[blockIdx.y].icgn;
+    if (P.rp == nullptr) return;
+    row_poly_rows(P.tar, P.stride, P.sx0, P.sy0, P.rw, P.rh, const_cast<uint4*>(P.rp), 4 * blockIdx.x,
+                  4 * gridDim.x);
+}
+
+hipError_t launch_batch_row_poly(const PairDesc* pairs, int npairs, int max_rh, hipStream_t st)
+{
+    if (npairs <= 0) return hipSuccess;
+    const int blocks = std::max(1, std::min((max_rh + 1 + 3) / 4, 1024));
+    hipLaunchKernelGGL(k_batch_row_poly, dim3(blocks, npairs), dim3(256), 0, st, pairs);
     return hipGetLastError();
 }
 

@@ -305,9 +305,13 @@ bool admit(twdic_server* s, Slot& sl, Job& j)
         build = true;
         s->st.state_builds++;
     }
+    // the frame, its row polynomials (the fused interpolation's, as the
+    // single context builds them) and the outputs
     const size_t tar_b = align_up((size_t)w * h);
+    const bool rp_fits = row_poly_fits(1, 1, w - 2, h - 2);
+    const size_t rp_b = rp_fits ? align_up(row_poly_bytes(w - 2, h - 2)) : 0;
     size_t off;
-    if (!alloc_evict(s, tar_b + out_bytes(n), &off, e)) {
+    if (!alloc_evict(s, tar_b + rp_b + out_bytes(n), &off, e)) {
         if (build) {      // undo: nothing was issued for it yet
             s->states.erase(key);
             s->arena.release(e->off, e->bytes);
@@ -322,7 +326,7 @@ bool admit(twdic_server* s, Slot& sl, Job& j)
     sl.job = j;
     sl.state = e;
     sl.off = off;
-    sl.bytes = tar_b + out_bytes(n);
+    sl.bytes = tar_b + rp_b + out_bytes(n);
     sl.n = n;
     s->busy_now++;
     s->st.admitted++;
@@ -330,7 +334,8 @@ bool admit(twdic_server* s, Slot& sl, Job& j)
 
     char* base = s->d_arena;
     uint8_t* d_tar = (uint8_t*)(base + off);
-    char* d_out = base + off + tar_b;
+    uint4* d_rp = rp_fits ? (uint4*)(base + off + tar_b) : nullptr;
+    char* d_out = base + off + tar_b + rp_b;
     float* d_u = (float*)d_out;
     float* d_v = d_u + n;
     float* d_z = d_v + n;
@@ -356,7 +361,7 @@ bool admit(twdic_server* s, Slot& sl, Job& j)
     ip.sx0 = 1; ip.sy0 = 1; ip.rw = w - 2; ip.rh = h - 2;
     ip.Sx = ip.Sy = r.subset; ip.Wsub = ip.Hsub = 2 * r.subset + 1; ip.s = ip.Wsub * ip.Hsub;
     ip.max_iter = r.max_iter; ip.delta_p = r.delta_p;
-    ip.g2 = d_g2; ip.lut = nullptr; ip.refstate = d_state;
+    ip.g2 = d_g2; ip.lut = nullptr; ip.rp = d_rp; ip.refstate = d_state;
     ip.poi = d_poi; ip.n = n;
     ip.u = d_u; ip.v = d_v; ip.p6 = d_p6; ip.iters = d_it; ip.flags = d_fl; ip.dpnorm = nullptr;
     if (build) {
@@ -371,6 +376,7 @@ bool admit(twdic_server* s, Slot& sl, Job& j)
     fp.u = d_u; fp.v = d_v; fp.zncc = d_z; fp.status = nullptr;
     if (!fp.tw) chk(hipErrorOutOfMemory);
     if (err == hipSuccess) chk(launch_fftcc(fp, kFftccDftMfma, sl.st));
+    if (err == hipSuccess && d_rp) chk(launch_row_poly(d_tar, w, 1, 1, w - 2, h - 2, d_rp, sl.st));
     if (err == hipSuccess) chk(launch_icgn(ip, TWDIC_INTERP_FUSED, 5, sl.st));
     chk(hipMemcpyAsync(sl.h_out, d_out, out_bytes(n), hipMemcpyDeviceToHost, sl.st));
     chk(hipEventRecord(sl.done, sl.st));

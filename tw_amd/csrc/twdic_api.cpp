@@ -189,6 +189,7 @@ IcgnParams icgn_params(twdic_ctx* c, const int32_t* poi, int n, float* u, float*
     p.Sx = g.sx; p.Sy = g.sy; p.Wsub = 2 * g.sx + 1; p.Hsub = 2 * g.sy + 1; p.s = p.Wsub * p.Hsub;
     p.max_iter = g.max_iter; p.delta_p = g.delta_p;
     p.g2 = c->d_g2; p.lut = c->d_lut;
+    p.rp = g.interp == TWDIC_INTERP_FUSED ? c->d_rp : nullptr;    // null when the plane does not fit
     p.refstate = c->d_refstate;
     p.poi = poi; p.n = n;
     p.u = u; p.v = v; p.p6 = p6; p.iters = iters; p.flags = flags; p.dpnorm = dpnorm;
@@ -199,24 +200,42 @@ IcgnParams icgn_params(twdic_ctx* c, const int32_t* poi, int n, float* u, float*
 
 namespace {
 
+// The per-target interpolation data of the context's mode, on the context
+// stream: the LUT (LUT, B-spline modes) or the row polynomials (fused mode).
+// Built once per target; every call rebuilds it once the target buffer has
+// been handed out, since the caller may then write it directly.
+hipError_t launch_target_interp(twdic_ctx* c, hipStream_t st)
+{
+    const twdic_config& g = c->cfg;
+    if (g.interp == TWDIC_INTERP_LUT)
+        // cuBicubicCoefficients, TW_paDIC_cuICGN2D.cu:844-851
+        return launch_bicubic_lut(c->d_tar, c->dstride, g.roi_x, g.roi_y, g.roi_w, g.roi_h, c->d_lut, st);
+    if (g.interp == TWDIC_INTERP_BSPLINE)
+        // BicubicSplineCoefficients_s, TW_paDIC_ICGN2D_CPU.cpp:333-340
+        return launch_bspline_lut(c->d_tar, c->dstride, g.roi_x, g.roi_y, g.roi_w, g.roi_h, c->d_lut, st);
+    if (c->d_rp) return launch_row_poly(c->d_tar, c->dstride, g.roi_x, g.roi_y, g.roi_w, g.roi_h, c->d_rp, st);
+    return hipSuccess;
+}
+
+twdic_status target_interp_prepare(twdic_ctx* c)
+{
+    bool& valid = c->cfg.interp == TWDIC_INTERP_FUSED ? c->tar_rp_valid : c->tar_pre_valid;
+    if (valid && !c->tar_exposed) return TWDIC_OK;
+    TW_HIP(c, launch_target_interp(c, c->stream));
+    valid = true;
+    return TWDIC_OK;
+}
+
 // Everything IC-GN needs before its kernel: reference gradients, the target
-// LUT in the LUT/B-spline modes, and the per-POI reference state, all on the
-// context stream.
+// LUT in the LUT/B-spline modes or the row polynomials in the fused mode, and
+// the per-POI reference state, all on the context stream.
 twdic_status icgn_prepare(twdic_ctx* c, const int32_t* d_poi, int n, float* d_u, float* d_v, float* d_p6,
                           int32_t* d_iters, uint8_t* d_flags, float* d_dpnorm, IcgnParams& p)
 {
-    const twdic_config& g = c->cfg;
     twdic_status st = ensure_ref_grads(c);
     if (st != TWDIC_OK) return st;
-    if (g.interp == TWDIC_INTERP_LUT && !c->tar_pre_valid) {
-        // per-frame target LUT (cuBicubicCoefficients, TW_paDIC_cuICGN2D.cu:844-851)
-        TW_HIP(c, launch_bicubic_lut(c->d_tar, c->dstride, g.roi_x, g.roi_y, g.roi_w, g.roi_h, c->d_lut, c->stream));
-        c->tar_pre_valid = true;
-    } else if (g.interp == TWDIC_INTERP_BSPLINE && !c->tar_pre_valid) {
-        // per-frame B-spline LUT (BicubicSplineCoefficients_s, TW_paDIC_ICGN2D_CPU.cpp:333-340)
-        TW_HIP(c, launch_bspline_lut(c->d_tar, c->dstride, g.roi_x, g.roi_y, g.roi_w, g.roi_h, c->d_lut, c->stream));
-        c->tar_pre_valid = true;
-    }
+    st = target_interp_prepare(c);
+    if (st != TWDIC_OK) return st;
     if ((size_t)n > c->refstate_cap) {
         if (c->d_refstate) {
             TW_HIP(c, hipStreamSynchronize(c->stream));
@@ -343,6 +362,13 @@ twdic_status twdic_create(const twdic_config* cfg, int device, twdic_ctx** out)
         if ((e = hipMalloc(a.p, a.b)) != hipSuccess) bail(e, "hipMalloc");
         else if ((e = hipMemset(*a.p, 0, a.b)) != hipSuccess) bail(e, "hipMemset");
     }
+    // fused mode: the target's row polynomials (16 bytes per ROI pixel);
+    // without them (ROIs past 2^28 pixels) IC-GN reads the image bytes
+    if (st == TWDIC_OK && cfg->interp == TWDIC_INTERP_FUSED &&
+        row_poly_fits(cfg->roi_x, cfg->roi_y, cfg->roi_w, cfg->roi_h)) {
+        if ((e = hipMalloc((void**)&c->d_rp, row_poly_bytes(cfg->roi_w, cfg->roi_h))) != hipSuccess)
+            bail(e, "hipMalloc");
+    }
     for (int i = 0; i < 16 && st == TWDIC_OK; ++i)
         if ((e = hipEventCreate(&c->ev[i])) != hipSuccess) bail(e, "hipEventCreate");
     for (int i = 0; i <= twdic_ctx::kMaxFrameChunks && st == TWDIC_OK; ++i)
@@ -374,7 +400,7 @@ twdic_status twdic_destroy(twdic_ctx* c)
     if (c->stream2) (void)hipStreamSynchronize(c->stream2);
     drop_graph(c);
     void* bufs[] = {c->d_ref, c->d_tar, c->d_rx, c->d_ry, c->d_tx, c->d_ty, c->d_txy, c->d_lut,
-                    c->d_fft_tw, c->d_refstate, c->d_g2};
+                    c->d_fft_tw, c->d_refstate, c->d_g2, c->d_rp};
     for (void* b : bufs)
         if (b) (void)hipFree(b);
     if (c->h_stage) (void)hipHostFree(c->h_stage);
@@ -423,6 +449,7 @@ twdic_status twdic_set_target(twdic_ctx* c, const uint8_t* img, int stride, int 
     if (st != TWDIC_OK) return st;
     c->tar_loaded = true;
     c->tar_pre_valid = false;
+    c->tar_rp_valid = false;
     return TWDIC_OK;
 }
 
@@ -539,14 +566,8 @@ twdic_status twdic_icgn_noalloc(twdic_ctx* c, const int32_t* d_poi, int n, float
     if (st != TWDIC_OK) return st;
     st = ensure_ref_grads(c);
     if (st != TWDIC_OK) return st;
-    const twdic_config& g = c->cfg;
-    if (g.interp == TWDIC_INTERP_LUT && !c->tar_pre_valid) {
-        TW_HIP(c, launch_bicubic_lut(c->d_tar, c->dstride, g.roi_x, g.roi_y, g.roi_w, g.roi_h, c->d_lut, c->stream));
-        c->tar_pre_valid = true;
-    } else if (g.interp == TWDIC_INTERP_BSPLINE && !c->tar_pre_valid) {
-        TW_HIP(c, launch_bspline_lut(c->d_tar, c->dstride, g.roi_x, g.roi_y, g.roi_w, g.roi_h, c->d_lut, c->stream));
-        c->tar_pre_valid = true;
-    }
+    st = target_interp_prepare(c);
+    if (st != TWDIC_OK) return st;
     // the scratch holds whatever the last call on this GPU left: the state is
     // rebuilt for every POI of this call (each POI's wavefront writes every
     // field IC-GN reads), then the iterations read it, in stream order
@@ -614,14 +635,10 @@ twdic_status twdic_frame_capture(twdic_ctx* c, const int32_t* d_poi, int n, floa
     twdic_status st = icgn_prepare(c, d_poi, n, d_u, d_v, d_p6, d_iters, d_flags, nullptr, ip);
     if (st != TWDIC_OK) return st;
     const FftccParams fp = fftcc_params(c, d_poi, n, d_u, d_v, d_zncc, nullptr);
-    const twdic_config& g = c->cfg;
     drop_graph(c);
     TW_HIP(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-    hipError_t e = hipSuccess;
-    if (g.interp == TWDIC_INTERP_LUT)
-        e = launch_bicubic_lut(c->d_tar, c->dstride, g.roi_x, g.roi_y, g.roi_w, g.roi_h, c->d_lut, c->stream);
-    else if (g.interp == TWDIC_INTERP_BSPLINE)
-        e = launch_bspline_lut(c->d_tar, c->dstride, g.roi_x, g.roi_y, g.roi_w, g.roi_h, c->d_lut, c->stream);
+    // the per-target interpolation data is part of every replay
+    hipError_t e = launch_target_interp(c, c->stream);
     if (e == hipSuccess) e = launch_fftcc(fp, c->fftcc_variant, c->stream);
     if (e == hipSuccess) e = launch_icgn(ip, c->cfg.interp, c->icgn_variant, c->stream);
     hipGraph_t gr = nullptr;
@@ -657,7 +674,8 @@ twdic_status twdic_frame_replay(twdic_ctx* c)
         c->refstate_n = ip.n;
     }
     TW_HIP(c, hipGraphLaunch(c->graph_exec, c->stream));
-    if (c->cfg.interp != TWDIC_INTERP_FUSED) c->tar_pre_valid = true;   // the graph rebuilt the LUT
+    // the graph rebuilt the LUT or the row polynomials
+    (c->cfg.interp == TWDIC_INTERP_FUSED ? c->tar_rp_valid : c->tar_pre_valid) = true;
     return TWDIC_OK;
 }
 
@@ -676,7 +694,7 @@ twdic_status twdic_get_buffer(twdic_ctx* c, int which, void** d_ptr, size_t* byt
     case TWDIC_BUF_TXY: p = c->d_txy; break;
     case TWDIC_BUF_LUT: p = c->d_lut; b = roi * 64; break;
     case TWDIC_BUF_REF: p = c->d_ref; b = img; break;
-    case TWDIC_BUF_TAR: p = c->d_tar; b = img; break;
+    case TWDIC_BUF_TAR: p = c->d_tar; b = img; c->tar_exposed = true; break;
     default: return fail(c, TWDIC_EINVAL, "unknown buffer");
     }
     if (which == TWDIC_BUF_RX || which == TWDIC_BUF_RY) {

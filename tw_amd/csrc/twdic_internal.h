@@ -69,6 +69,7 @@ struct IcgnParams {
     float delta_p;
     const uint32_t* g2;       // reference gradients (2Rx, 2Ry) as int16 pairs, ROI-relative
     const float* lut;         // target LUT [rh][rw][16]
+    const uint4* rp;          // target row polynomials [rh + 1][rw] (fused mode; null: from the image bytes)
     double* refstate;         // kRefStateStride doubles per POI
     const int32_t* poi;
     int n;
@@ -106,6 +107,12 @@ hipError_t launch_gradients(const uint8_t* img, int stride, int sx0, int sy0, in
                             float* gx, float* gy, float* gxy, uint32_t* g2, hipStream_t st);
 hipError_t launch_bspline_lut(const uint8_t* tar, int stride, int sx0, int sy0, int rw, int rh, float* lut,
                               hipStream_t st);
+// fp16 Catmull-Rom row polynomials of the target, (rh + 1) x rw entries of 16 bytes
+hipError_t launch_row_poly(const uint8_t* tar, int stride, int sx0, int sy0, int rw, int rh, uint4* rp,
+                           hipStream_t st);
+// the same for every pair of a batch with a plane (descriptor icgn.rp set);
+// max_rh: the largest ROI height among them
+hipError_t launch_batch_row_poly(const PairDesc* pairs, int npairs, int max_rh, hipStream_t st);
 hipError_t launch_bicubic_lut(const uint8_t* tar, int stride, int sx0, int sy0, int rw, int rh,
                               float* lut, hipStream_t st);
 hipError_t launch_poi_grid(int32_t* poi, int start_x, int start_y, int nx, int ny, int mx, int my,
@@ -129,6 +136,17 @@ hipError_t launch_batch_fftcc_mma(const PairDesc* pairs, const int2* items, int 
 // POI), four POIs of one pair per workgroup; `table` when every listed pair's
 // subset fits the step table (step_table_fits).
 bool step_table_fits(int wsub, int hsub);
+// The row-polynomial plane of an ROI is addressed with 32-bit byte offsets
+// 16 (iy rw + ix) from entry -(sy0 rw + sx0); true when they all fit.
+inline bool row_poly_fits(int sx0, int sy0, int rw, int rh)
+{
+    return 16ull * ((unsigned long long)(sy0 + rh + 2) * (unsigned long long)rw + (unsigned long long)sx0) <
+           (1ull << 32);
+}
+inline size_t row_poly_bytes(int rw, int rh)
+{
+    return 16ull * (size_t)rw * (size_t)(rh + 1);
+}
 // waves: occupancy target 5 or 6; xcd_ranges: contiguous per-XCD item ranges
 hipError_t launch_batch_icgn_wg(const PairDesc* pairs, const int2* wg, int count, bool table, int waves,
                                 bool xcd_ranges, hipStream_t st);
@@ -173,11 +191,14 @@ struct twdic_ctx {
     float* d_ty = nullptr;
     float* d_txy = nullptr;
     float* d_lut = nullptr;
+    uint4* d_rp = nullptr;            // target row polynomials (fused interpolation)
     uint32_t* d_g2 = nullptr;         // packed (2Rx, 2Ry) int16 pairs
     float2* d_fft_tw = nullptr;
     bool ref_loaded = false, tar_loaded = false;
     bool ref_grads_valid = false;     // Rx/Ry match d_ref
     bool tar_pre_valid = false;       // Tx/Ty/Txy/LUT match d_tar
+    bool tar_rp_valid = false;        // d_rp matches d_tar
+    bool tar_exposed = false;         // d_tar was handed out: it may change without twdic_set_target
     uint64_t ref_version = 0;
     // per-reference POI state cache key
     double* d_refstate = nullptr;
