@@ -41,6 +41,7 @@ struct twdic_batch {
     uint32_t* d_g2 = nullptr;         // reference gradient planes of all pairs
     uint4* d_rp = nullptr;            // target row polynomials of all pairs (rebuilt every step)
     int max_rh = 0;                   // largest ROI height among the pairs with a plane
+    bool all_rp = false;              // every pair has a plane
     double* d_state = nullptr;        // per-POI reference state of all pairs
     float* d_zncc_scratch = nullptr;  // for pairs without a ZNCC output
     std::map<int, float2*> tw;        // FFT twiddle tables per window edge
@@ -118,7 +119,7 @@ void release(twdic_batch* b)
     b->d_desc = nullptr; b->d_items = nullptr;
     b->d_fitems[0] = b->d_fitems[1] = b->d_fitems[2] = nullptr;
     b->d_g2 = nullptr; b->d_state = nullptr; b->d_zncc_scratch = nullptr; b->d_counters = nullptr;
-    b->d_rp = nullptr; b->max_rh = 0;
+    b->d_rp = nullptr; b->max_rh = 0; b->all_rp = false;
     b->bytes = 0;
 }
 
@@ -270,7 +271,7 @@ twdic_status enqueue_flat(twdic_batch* b, bool fftcc, bool icgn)
         for (int t = 0; t < 2; ++t)
             if (b->nwg[t])
                 TB_HIP(b, launch_batch_icgn_wg(b->d_desc, b->d_wg[t], (int)b->nwg[t], t == 0, b->tune.icgn_waves,
-                                               b->tune.icgn_xcd_ranges != 0, b->stream));
+                                               b->tune.icgn_xcd_ranges != 0, b->all_rp, b->stream));
     return TWDIC_OK;
 }
 
@@ -354,6 +355,7 @@ twdic_status set_pairs_impl(twdic_batch* b, const twdic_pair* pairs, int npairs)
     std::vector<int2> fitems[3];
     items.reserve(total);
     size_t g2_off = 0, st_off = 0, zn_off = 0, rp_off = 0;
+    b->all_rp = true;
     for (int i = 0; i < npairs; ++i) {
         const twdic_pair& p = b->pairs[i];
         if (!b->tw.count(p.fft_n)) {
@@ -380,6 +382,8 @@ twdic_status set_pairs_impl(twdic_batch* b, const twdic_pair* pairs, int npairs)
             q.rp = b->d_rp + rp_off;
             rp_off += row_poly_bytes(p.roi_w, p.roi_h) / 16;
             b->max_rh = std::max(b->max_rh, p.roi_h);
+        } else {
+            b->all_rp = false;
         }
         q.refstate = b->d_state + st_off;
         q.poi = p.d_poi_yx; q.n = p.n;

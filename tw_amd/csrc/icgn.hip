@@ -550,7 +550,10 @@ __global__ __launch_bounds__(256, MINW > 0 ? MINW : 1) void k_icgn(IcgnParams P)
 // table is that pair's.  XCD: items in contiguous per-XCD ranges as in
 // k_icgn (one band of the work per L2); otherwise dealt out in block order,
 // which balances the XCDs when the work per item differs from pair to pair.
-template <int INTERP, int MINW, bool TABLE, bool XCD>
+// ALL_RP: every pair has its row-polynomial plane (the usual case), so only
+// that form is compiled in and gets the register budget; otherwise each pair
+// takes its form at run time.
+template <int INTERP, int MINW, bool TABLE, bool XCD, bool ALL_RP>
 __global__ __launch_bounds__(256, MINW > 0 ? MINW : 1) void k_batch_icgn_wg(const PairDesc* __restrict__ pairs,
                                                                            const int2* __restrict__ wg)
 {
@@ -562,10 +565,16 @@ __global__ __launch_bounds__(256, MINW > 0 ? MINW : 1) void k_batch_icgn_wg(cons
         fill_step_table(tab, P);
         __syncthreads();
         if (poi >= P.n) return;
-        icgn_poi<INTERP, true>(P, poi, threadIdx.x & 63, &tab);
+        if constexpr (ALL_RP)
+            icgn_poi_t<INTERP, true, true>(P, poi, threadIdx.x & 63, &tab);
+        else
+            icgn_poi<INTERP, true>(P, poi, threadIdx.x & 63, &tab);
     } else {
         if (poi >= P.n) return;
-        icgn_poi<INTERP, false>(P, poi, threadIdx.x & 63, nullptr);
+        if constexpr (ALL_RP)
+            icgn_poi_t<INTERP, false, true>(P, poi, threadIdx.x & 63, nullptr);
+        else
+            icgn_poi<INTERP, false>(P, poi, threadIdx.x & 63, nullptr);
     }
 }
 
@@ -574,29 +583,34 @@ bool step_table_fits(int wsub, int hsub)
     return wsub * hsub <= kStepTable;
 }
 
-template <int MINW, bool TABLE, bool XCD>
+template <int MINW, bool TABLE, bool XCD, bool ALL_RP>
 static void launch_wg(const PairDesc* pairs, const int2* wg, int count, hipStream_t st)
 {
-    hipLaunchKernelGGL((k_batch_icgn_wg<TWDIC_INTERP_FUSED, MINW, TABLE, XCD>), dim3(count), dim3(256), 0, st, pairs, wg);
+    hipLaunchKernelGGL((k_batch_icgn_wg<TWDIC_INTERP_FUSED, MINW, TABLE, XCD, ALL_RP>), dim3(count), dim3(256), 0, st,
+                       pairs, wg);
 }
 
-template <int MINW>
+template <int MINW, bool ALL_RP>
 static void launch_wg_w(const PairDesc* pairs, const int2* wg, int count, bool table, bool xcd, hipStream_t st)
 {
     if (table)
-        xcd ? launch_wg<MINW, true, true>(pairs, wg, count, st) : launch_wg<MINW, true, false>(pairs, wg, count, st);
+        xcd ? launch_wg<MINW, true, true, ALL_RP>(pairs, wg, count, st)
+            : launch_wg<MINW, true, false, ALL_RP>(pairs, wg, count, st);
     else
-        xcd ? launch_wg<MINW, false, true>(pairs, wg, count, st) : launch_wg<MINW, false, false>(pairs, wg, count, st);
+        xcd ? launch_wg<MINW, false, true, ALL_RP>(pairs, wg, count, st)
+            : launch_wg<MINW, false, false, ALL_RP>(pairs, wg, count, st);
 }
 
 hipError_t launch_batch_icgn_wg(const PairDesc* pairs, const int2* wg, int count, bool table, int waves,
-                                bool xcd_ranges, hipStream_t st)
+                                bool xcd_ranges, bool all_rp, hipStream_t st)
 {
     if (count <= 0) return hipSuccess;
-    if (waves == 6)
-        launch_wg_w<6>(pairs, wg, count, table, xcd_ranges, st);
+    if (all_rp)
+        waves == 6 ? launch_wg_w<6, true>(pairs, wg, count, table, xcd_ranges, st)
+                   : launch_wg_w<5, true>(pairs, wg, count, table, xcd_ranges, st);
     else
-        launch_wg_w<5>(pairs, wg, count, table, xcd_ranges, st);
+        waves == 6 ? launch_wg_w<6, false>(pairs, wg, count, table, xcd_ranges, st)
+                   : launch_wg_w<5, false>(pairs, wg, count, table, xcd_ranges, st);
     return hipGetLastError();
 }
 

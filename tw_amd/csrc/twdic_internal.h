@@ -147,9 +147,10 @@ inline size_t row_poly_bytes(int rw, int rh)
 {
     return 16ull * (size_t)rw * (size_t)(rh + 1);
 }
-// waves: occupancy target 5 or 6; xcd_ranges: contiguous per-XCD item ranges
+// waves: occupancy target 5 or 6; xcd_ranges: contiguous per-XCD item ranges;
+// all_rp: every pair's descriptor has its row-polynomial plane
 hipError_t launch_batch_icgn_wg(const PairDesc* pairs, const int2* wg, int count, bool table, int waves,
-                                bool xcd_ranges, hipStream_t st);
+                                bool xcd_ranges, bool all_rp, hipStream_t st);
 hipError_t launch_ref_state(const IcgnParams& p, hipStream_t st);
 hipError_t launch_icgn(const IcgnParams& p, int interp, int variant, hipStream_t st);
 // persistent batch kernels: `blocks` workgroups drain a device work counter
