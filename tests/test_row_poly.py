@@ -8,11 +8,14 @@ evaluates that spline as
     c0 = p1, c1 = p2 - p0, c2 = 2 p0 - 5 p1 + 4 p2 - p3, c3 = 3 (p1 - p2) + p3 - p0,
     T = R1 + ty/2 (d1 + ty (d2 + ty d3)),
     d1 = R2 - R0, d2 = 2 R0 - 5 R1 + 4 R2 - R3, d3 = 3 (R1 - R2) + R3 - R0,
-with the integer row coefficients stored as fp16 (precompute.hip k_row_poly;
-icgn.hip interp_row_poly).  These tests check, on the CPU:
+with the row coefficients stored as fp16 c0, c1/2, c2/2, c3/2 (precompute.hip
+k_row_poly; icgn.hip interp_row_poly evaluates c0 + tx (c1/2 + tx (c2/2 +
+tx c3/2)), which rounds exactly as the form above: every intermediate is half
+of it).  These tests check, on the CPU:
   * the form equals the oracle's LUT polynomial exactly (exact dyadic
     arithmetic in float64, so a wrong coefficient or sign fails);
-  * the coefficients fit fp16 exactly (|c| <= 1530 < 2048);
+  * the stored coefficients are exact in fp16 (|c| <= 1530, halves of
+    multiples of 1/2 below 766);
   * the FP32 evaluation in the kernel's operation order (every FMA rounded
     once, emulated exactly with fractions) returns a uniform neighbourhood's
     value exactly, which keeps the reference's DarkSubset behaviour on flat
@@ -75,8 +78,11 @@ def row_poly_f32(nb, tx, ty):
     R = []
     for k in range(4):
         c0, c1, c2, c3 = row_coeffs(nb[k])
-        r = fma32(fma32(tx, c3, c2), tx, c1)
-        R.append(fma32(hx, r, c0))
+        h1, h2, h3 = Fraction(c1, 2), Fraction(c2, 2), Fraction(c3, 2)
+        r = fma32(tx, fma32(tx, fma32(tx, h3, h2), h1), c0)
+        # the same value as the unhalved order with t/2 applied last
+        assert r == fma32(hx, fma32(fma32(tx, c3, c2), tx, c1), c0)
+        R.append(r)
     d1 = sub32(R[2], R[0])
     d2 = fma32(4, R[2], fma32(-5, R[1], fma32(2, R[0], -R[3])))
     d3 = fma32(3, sub32(R[1], R[2]), sub32(R[3], R[0]))
@@ -109,9 +115,10 @@ def test_row_coefficients_are_exact_in_fp16():
     rng = np.random.default_rng(1)
     rows = ext + [rng.integers(0, 256, 4) for _ in range(2000)]
     for r in rows:
-        for c in row_coeffs(r):
-            assert abs(c) <= 1530
-            assert float(np.float16(c)) == c
+        c = row_coeffs(r)
+        assert all(abs(v) <= 1530 for v in c)
+        for v in (c[0], c[1] / 2, c[2] / 2, c[3] / 2):
+            assert float(np.float16(v)) == v
 
 
 def test_fp32_order_is_exact_on_uniform_neighbourhoods_and_close_elsewhere():

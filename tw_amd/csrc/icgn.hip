@@ -176,13 +176,15 @@ __device__ __forceinline__ RowPolyBuf row_poly_buf(const IcgnParams& P)
 
 typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 
-// One row's cubic at t: c0 + (t/2) (c1 + t (c2 + t c3)), the fp16
-// coefficients taken as exact f32 operands by the mixed-precision FMA.
-__device__ __forceinline__ float row_poly(uint32_t c01, uint32_t c23, float t, float ht)
+// One row's cubic at t: c0 + t (c1/2 + t (c2/2 + t c3/2)), the fp16
+// coefficients (stored halved, exactly) taken as exact f32 operands by the
+// mixed-precision FMA.  Every intermediate is exactly half the one of
+// c0 + (t/2) (c1 + t (c2 + t c3)) evaluated in the same order, so the two
+// forms round identically.
+__device__ __forceinline__ float row_poly(uint32_t c01, uint32_t c23, float t)
 {
     const half2_t a = __builtin_bit_cast(half2_t, c01), b = __builtin_bit_cast(half2_t, c23);
-    const float r = fmaf(fmaf(t, (float)b.y, (float)b.x), t, (float)a.y);
-    return fmaf(ht, r, (float)a.x);
+    return fmaf(t, fmaf(t, fmaf(t, (float)b.y, (float)b.x), (float)a.y), (float)a.x);
 }
 
 // Bicubic value at (ix + tx, iy + ty): the four rows' cubics at tx, then the
@@ -196,9 +198,8 @@ __device__ __forceinline__ float interp_row_poly(const RowPolyBuf& B, int ix, in
     const int o = (int)__umul24((uint32_t)iy, (uint32_t)B.rw16) + 16 * ix;
     const u32x4 a = __builtin_amdgcn_raw_buffer_load_b128(B.r, o, 0, 0);
     const u32x4 b = __builtin_amdgcn_raw_buffer_load_b128(B.r, o, 2 * B.rw16, 0);
-    const float hx = 0.5f * tx;
-    const float R0 = row_poly(a.x, a.y, tx, hx), R1 = row_poly(a.z, a.w, tx, hx);
-    const float R2 = row_poly(b.x, b.y, tx, hx), R3 = row_poly(b.z, b.w, tx, hx);
+    const float R0 = row_poly(a.x, a.y, tx), R1 = row_poly(a.z, a.w, tx);
+    const float R2 = row_poly(b.x, b.y, tx), R3 = row_poly(b.z, b.w, tx);
     const float d1 = R2 - R0;
     const float d2 = fmaf(4.f, R2, fmaf(-5.f, R1, fmaf(2.f, R0, -R3)));
     const float d3 = fmaf(3.f, R1 - R2, R3 - R0);
@@ -250,10 +251,13 @@ __device__ __forceinline__ double conv_norm(const float d[6], int Sx, int Sy)
 // the gradient plane from the subset's corner.  Every POI of a launch has the
 // same subset and ROI width, so one table in LDS serves a workgroup's four
 // wavefronts, and a lane's step to its next pixel is an LDS read instead of
-// float compares and selects.
+// float compares and selects.  The three arrays have the same 4-byte stride,
+// so one address register reaches all of them (the array offsets are
+// immediates of the LDS reads).
 constexpr int kStepTable = 2048;          // pixels: subsets up to 45 x 45
 struct StepTable {
-    float2 d[kStepTable];
+    float dx[kStepTable];
+    float dy[kStepTable];
     uint32_t go[kStepTable];
 };
 
@@ -261,7 +265,8 @@ __device__ __forceinline__ void fill_step_table(StepTable& t, const IcgnParams& 
 {
     for (int p = threadIdx.x; p < P.s; p += blockDim.x) {
         const int l = p / P.Wsub, m = p - l * P.Wsub;
-        t.d[p] = make_float2((float)(m - P.Sx), (float)(l - P.Sy));
+        t.dx[p] = (float)(m - P.Sx);
+        t.dy[p] = (float)(l - P.Sy);
         t.go[p] = 4u * (uint32_t)(l * P.rw + m);
     }
 }
@@ -340,13 +345,11 @@ __device__ __forceinline__ bool subset_loop(const IcgnParams& P, int px, int py,
 #pragma unroll 1
         for (int k = 0; k < full; ++k) {
             const int p = lane + 64 * k;
-            const float2 d = tab->d[p];
-            pixel(d.x, d.y, tab->go[p], gcorner);
+            pixel(tab->dx[p], tab->dy[p], tab->go[p], gcorner);
         }
         const int p = lane + 64 * full;
         if (p < s) {
-            const float2 d = tab->d[p];
-            pixel(d.x, d.y, tab->go[p], gcorner);
+            pixel(tab->dx[p], tab->dy[p], tab->go[p], gcorner);
         }
     } else {
         // lane walks pixels p = lane, lane + 64, ...: column m = p % Wsub, row

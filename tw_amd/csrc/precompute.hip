@@ -224,16 +224,18 @@ hipError_t launch_bicubic_lut(const uint8_t* tar, int stride, int sx0, int sy0, 
 // and x + 1, its cubic through p0..p3 = T(y, x-1 .. x+2) is
 //   R(t) = c0 + t/2 (c1 + t (c2 + t c3)),
 //   c0 = p1, c1 = p2 - p0, c2 = 2 p0 - 5 p1 + 4 p2 - p3, c3 = 3 (p1 - p2) + p3 - p0,
-// integers of magnitude at most 1530, so exact in fp16.  Entry (x, j) holds
-// the four coefficients of rows y = sy0 - 1 + j and y + 1 (fp16, c0..c3 of
-// row y, then of row y + 1: 16 bytes) for cell column sx0 + x, x < rw,
-// j <= rh; the last column (never an interior cell) is zero.  A cell's four
-// neighbourhood rows are then two 16-byte entries, (x, j) and (x, j + 2).
+// integers of magnitude at most 1530.  Stored as c0, c1/2, c2/2, c3/2 (the
+// halves are multiples of 1/2 below 766 in magnitude, so exact in fp16), the
+// cubic is c0 + t (c1/2 + t (c2/2 + t c3/2)).  Entry (x, j) holds the four
+// coefficients of rows y = sy0 - 1 + j and y + 1 (fp16, row y, then row y + 1:
+// 16 bytes) for cell column sx0 + x, x < rw, j <= rh; the last column (never
+// an interior cell) is zero.  A cell's four neighbourhood rows are then two
+// 16-byte entries, (x, j) and (x, j + 2).
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ uint32_t h2bits(int lo, int hi)
+__device__ __forceinline__ uint32_t h2bits(float lo, float hi)
 {
-    const uint32_t a = __builtin_bit_cast(uint16_t, (_Float16)(float)lo);
-    const uint32_t b = __builtin_bit_cast(uint16_t, (_Float16)(float)hi);
+    const uint32_t a = __builtin_bit_cast(uint16_t, (_Float16)lo);
+    const uint32_t b = __builtin_bit_cast(uint16_t, (_Float16)hi);
     return a | (b << 16);
 }
 
@@ -254,8 +256,9 @@ __device__ __forceinline__ void row_poly_rows(const uint8_t* __restrict__ tar, i
                 for (int r = 0; r < 2; ++r) {
                     const uint8_t* q = b + (size_t)r * stride + x;
                     const int p0 = q[0], p1 = q[1], p2 = q[2], p3 = q[3];
-                    h[2 * r] = h2bits(p1, p2 - p0);
-                    h[2 * r + 1] = h2bits(2 * p0 - 5 * p1 + 4 * p2 - p3, 3 * (p1 - p2) + p3 - p0);
+                    h[2 * r] = h2bits((float)p1, 0.5f * (float)(p2 - p0));
+                    h[2 * r + 1] = h2bits(0.5f * (float)(2 * p0 - 5 * p1 + 4 * p2 - p3),
+                                          0.5f * (float)(3 * (p1 - p2) + p3 - p0));
                 }
                 e = make_uint4(h[0], h[1], h[2], h[3]);
             }
