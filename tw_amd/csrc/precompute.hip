@@ -239,63 +239,72 @@ __device__ __forceinline__ uint32_t h2bits(float lo, float hi)
     return a | (b << 16);
 }
 
-// Entry rows j0, j0 + jstep, ... (four per 256-thread block, one per
-// wavefront) of one ROI's plane.
-__device__ __forceinline__ void row_poly_rows(const uint8_t* __restrict__ tar, int stride, int sx0, int sy0, int rw,
-                                              int rh, uint4* __restrict__ rp, int j0, int jstep)
+// Entry (x, j) from the two target rows' dwords at columns sx0 + x - 1 ..
+// sx0 + x + 2: for x < rw - 1 they lie in the image (the ROI keeps a one-pixel
+// frame); the buffer's range check would read anything past its last row as
+// zero.  The target resource's base is the image, records (sy0 + rh + 1) rows.
+__device__ __forceinline__ void row_poly_entry(__amdgpu_buffer_rsrc_t img, int stride, int sx0, int sy0, int rw,
+                                               int x, int j, uint4* __restrict__ rp)
 {
-    for (int j = j0 + (int)(threadIdx.x >> 6); j <= rh; j += jstep) {
-        const uint8_t* b = tar + (size_t)(sy0 - 1 + j) * stride + (sx0 - 1);
-        for (int x = threadIdx.x & 63; x < rw; x += 64) {
-            uint4 e = make_uint4(0u, 0u, 0u, 0u);
-            if (x < rw - 1) {
-                // rows sy0 - 1 + j, +1 and columns sx0 + x - 1 .. sx0 + x + 2
-                // lie in the image: the ROI keeps a one-pixel frame
-                uint32_t h[4];
+    uint4 e = make_uint4(0u, 0u, 0u, 0u);
+    if (x < rw - 1) {
+        const int o = (sy0 - 1 + j) * stride + (sx0 - 1 + x);
+        uint32_t h[4];
 #pragma unroll
-                for (int r = 0; r < 2; ++r) {
-                    const uint8_t* q = b + (size_t)r * stride + x;
-                    const int p0 = q[0], p1 = q[1], p2 = q[2], p3 = q[3];
-                    h[2 * r] = h2bits((float)p1, 0.5f * (float)(p2 - p0));
-                    h[2 * r + 1] = h2bits(0.5f * (float)(2 * p0 - 5 * p1 + 4 * p2 - p3),
-                                          0.5f * (float)(3 * (p1 - p2) + p3 - p0));
-                }
-                e = make_uint4(h[0], h[1], h[2], h[3]);
-            }
-            rp[(size_t)j * rw + x] = e;
+        for (int r = 0; r < 2; ++r) {
+            const uint32_t w = __builtin_amdgcn_raw_buffer_load_b32(img, o, r * stride, 0);
+            const int p0 = (int)(w & 255u), p1 = (int)((w >> 8) & 255u), p2 = (int)((w >> 16) & 255u),
+                      p3 = (int)(w >> 24);
+            h[2 * r] = h2bits((float)p1, 0.5f * (float)(p2 - p0));
+            h[2 * r + 1] = h2bits(0.5f * (float)(2 * p0 - 5 * p1 + 4 * p2 - p3), 0.5f * (float)(3 * (p1 - p2) + p3 - p0));
         }
+        e = make_uint4(h[0], h[1], h[2], h[3]);
     }
+    rp[(size_t)j * rw + x] = e;
 }
 
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t row_poly_src(const uint8_t* tar, int stride, int sy0, int rh)
+{
+    return __builtin_amdgcn_make_buffer_rsrc((void*)tar, (short)0, (sy0 + rh + 1) * stride, 0x00020000);
+}
+
+// One entry per thread: x along the block's 256 threads and blockIdx.x, rows
+// j = blockIdx.y, + gridDim.y, ...
 __global__ __launch_bounds__(256) void k_row_poly(const uint8_t* __restrict__ tar, int stride, int sx0, int sy0,
                                                   int rw, int rh, uint4* __restrict__ rp)
 {
-    row_poly_rows(tar, stride, sx0, sy0, rw, rh, rp, 4 * blockIdx.x, 4 * gridDim.x);
+    const __amdgpu_buffer_rsrc_t img = row_poly_src(tar, stride, sy0, rh);
+    const int x = blockIdx.x * 256 + threadIdx.x;
+    if (x >= rw) return;
+    for (int j = blockIdx.y; j <= rh; j += gridDim.y) row_poly_entry(img, stride, sx0, sy0, rw, x, j, rp);
 }
 
 hipError_t launch_row_poly(const uint8_t* tar, int stride, int sx0, int sy0, int rw, int rh, uint4* rp,
                            hipStream_t st)
 {
-    const int blocks = std::min((rh + 1 + 3) / 4, 4096);
-    hipLaunchKernelGGL(k_row_poly, dim3(blocks), dim3(256), 0, st, tar, stride, sx0, sy0, rw, rh, rp);
+    const dim3 grid((rw + 255) / 256, std::min(rh + 1, 65535));
+    hipLaunchKernelGGL(k_row_poly, grid, dim3(256), 0, st, tar, stride, sx0, sy0, rw, rh, rp);
     return hipGetLastError();
 }
 
 // Batched form: blockIdx.y is the pair; geometry and buffers come from its
 // descriptor (read when the kernel runs, so a captured graph follows new
-// targets).  Pairs without a plane (rp null) are skipped.
+// targets).  Rows j = blockIdx.x, + gridDim.x, ..., the block's threads along
+// x.  Pairs without a plane (rp null) are skipped.
 __global__ __launch_bounds__(256) void k_batch_row_poly(const PairDesc* __restrict__ pairs)
 {
     const IcgnParams P = const_pairs(pairs)[blockIdx.y].icgn;
     if (P.rp == nullptr) return;
-    row_poly_rows(P.tar, P.stride, P.sx0, P.sy0, P.rw, P.rh, const_cast<uint4*>(P.rp), 4 * blockIdx.x,
-                  4 * gridDim.x);
+    const __amdgpu_buffer_rsrc_t img = row_poly_src(P.tar, P.stride, P.sy0, P.rh);
+    for (int j = blockIdx.x; j <= P.rh; j += gridDim.x)
+        for (int x = threadIdx.x; x < P.rw; x += 256)
+            row_poly_entry(img, P.stride, P.sx0, P.sy0, P.rw, x, j, const_cast<uint4*>(P.rp));
 }
 
 hipError_t launch_batch_row_poly(const PairDesc* pairs, int npairs, int max_rh, hipStream_t st)
 {
     if (npairs <= 0) return hipSuccess;
-    const int blocks = std::max(1, std::min((max_rh + 1 + 3) / 4, 1024));
+    const int blocks = std::max(1, std::min(max_rh + 1, 1024));
     hipLaunchKernelGGL(k_batch_row_poly, dim3(blocks, npairs), dim3(256), 0, st, pairs);
     return hipGetLastError();
 }
