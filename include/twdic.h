@@ -57,10 +57,13 @@ enum {
     TWDIC_INTERP_LUT = 0,   /* materialise the 16-coefficient bicubic LUT of the
                                target each frame and gather from it, as the
                                reference does (TW_cuUtils.cu:314-435) */
-    TWDIC_INTERP_FUSED = 1, /* evaluate the same bicubic interpolant directly
-                               from the target pixels (tensor Catmull-Rom, which
-                               equals the LUT polynomial built from central-
-                               difference derivatives); no LUT traffic */
+    TWDIC_INTERP_FUSED = 1, /* evaluate the same bicubic interpolant (tensor
+                               Catmull-Rom, which equals the LUT polynomial built
+                               from central-difference derivatives) from the
+                               target rows' cubic coefficients: integers stored
+                               exactly in fp16, 16 bytes per ROI pixel, built once
+                               per target (every call once TWDIC_BUF_TAR has been
+                               handed out), instead of the 64-byte LUT */
     TWDIC_INTERP_BSPLINE = 2 /* ICGN2DInterpolationFLag::BicubicSpline: the cubic
                                B-spline LUT of BicubicSplineCoefficients_s
                                (TW_utils.cpp:521-602), CPU-only in the reference
