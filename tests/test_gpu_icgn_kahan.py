@@ -1,0 +1,38 @@
+"""The IC-GN sums as north_star names them, FP32 with compensated (Kahan)
+addition (TWDIC_ICGN_VARIANT=20), held to the same parity bars as the
+default FP64 sums on the 512^2 workload.  The variant exists to measure that
+alternative against the default (on gfx950 an FP64 FMA costs the same issue
+slot as an unpacked FP32 one, and the compensated form needs about 32 more
+VALU instructions per pixel); the default stays FP64."""
+import numpy as np
+import pytest
+
+import oracle
+from tests import cases
+from tests.gpu_helpers import check_icgn, ctx_for, run_icgn
+
+pytestmark = pytest.mark.gpu
+
+
+def test_kahan_fp32_sums_meet_the_parity_bars(monkeypatch):
+    c = cases.speckle_512()
+    poi = c.poi()
+    f = oracle.fftcc(c.ref, c.tar, poi, 32)
+    o = oracle.icgn(c.ref, c.tar, c.roi, 15, 15, poi, f["u"], f["v"])
+    o64 = oracle.icgn(c.ref, c.tar, c.roi, 15, 15, poi, f["u"].astype(np.float64), f["v"].astype(np.float64),
+                      precision="f64")
+    print("[gpu-test] IC-GN with compensated FP32 sums (variant 20)", flush=True)
+    monkeypatch.setenv("TWDIC_ICGN_VARIANT", "20")
+    with ctx_for(c) as ctx:
+        g = run_icgn(ctx, poi, f["u"], f["v"])
+    monkeypatch.setenv("TWDIC_ICGN_VARIANT", "5")
+    with ctx_for(c) as ctx:
+        d = run_icgn(ctx, poi, f["u"], f["v"])
+    # the bars of test_gpu_parity.py::test_icgn_parity
+    check_icgn(g, o, gtol=3e-6)
+    assert check_icgn(g, o64, utol=5e-5, gtol=2e-6) <= 2
+    # and the two accumulations agree where they take the same number of steps
+    same = (g["iters"] == d["iters"]) & (g["flags"] == 0)
+    assert same.mean() > 0.99
+    du = np.maximum(np.abs(g["u"] - d["u"]), np.abs(g["v"] - d["v"]))[same]
+    assert du.max() < 1e-4, du.max()

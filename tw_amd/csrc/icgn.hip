@@ -246,6 +246,26 @@ __device__ __forceinline__ double conv_norm(const float d[6], int Sx, int Sy)
 
 }  // namespace
 
+// FP32 sums with compensated (Kahan) addition of each pixel's T, T^2 and
+// SD_k T / 2 terms, every product and sum rounded separately: the accumulation
+// north_star names, built as a measured alternative to the FP64 sums
+// (TWDIC_ICGN_VARIANT=20), not the default.  The FP32 products SD_k T round
+// here, where the FP64 ones are exact.
+__device__ __forceinline__ void kahan_pixel(float ks[8], float kc[8], float T, uint32_t g, float dx, float dy)
+{
+#pragma clang fp contract(off)
+    const float gx = (float)(int16_t)(g & 0xffffu), gy = (float)(int16_t)(g >> 16);
+    const float a = gx * T, b = gy * T;
+    const float x[8] = {T, T * T, a, a * dx, a * dy, b, b * dx, b * dy};
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        const float y = x[i] - kc[i];
+        const float t = ks[i] + y;
+        kc[i] = (t - ks[i]) - y;
+        ks[i] = t;
+    }
+}
+
 // The subset's pixels in row-major order, p = l * Wsub + m: the exact float
 // offsets dx = m - Sx, dy = l - Sy and the byte offset 4 (l * rw + m) into
 // the gradient plane from the subset's corner.  Every POI of a launch has the
@@ -280,7 +300,7 @@ __device__ __forceinline__ void fill_step_table(StepTable& t, const IcgnParams& 
 // offsets in registers.  Each lane visits the same pixels in the same order
 // either way, so the sums are identical.  RP: the fused bicubic passes
 // evaluate the target's row polynomials (P.rp) instead of the image bytes.
-template <int INTERP, bool INT, bool CHECK, bool TABLE, bool RP>
+template <int INTERP, bool INT, bool CHECK, bool TABLE, bool RP, bool K32 = false>
 __device__ __forceinline__ bool subset_loop(const IcgnParams& P, int px, int py, const float Wm[6], int lane,
                                             double acc[8], const StepTable* tab)
 {
@@ -295,6 +315,7 @@ __device__ __forceinline__ bool subset_loop(const IcgnParams& P, int px, int py,
     const RowPolyBuf rb = RP ? row_poly_buf(P) : RowPolyBuf{};
     const float fpx = (float)px, fpy = (float)py;
     double st = 0.0, stt = 0.0, a0 = 0.0, a1 = 0.0, a2 = 0.0, b0 = 0.0, b1 = 0.0, b2 = 0.0;
+    float ks[8] = {}, kc[8] = {};
     bool out = false;
     auto pixel = [&](float dx, float dy, uint32_t go, int goff) {
         const float wx = warp_coord(fpx, Wm[0], Wm[1], Wm[2], dx, dy);
@@ -322,6 +343,10 @@ __device__ __forceinline__ bool subset_loop(const IcgnParams& P, int px, int py,
             }
         }
         const uint32_t g = __builtin_amdgcn_raw_buffer_load_b32(g2r, (int)go, goff, 0);
+        if constexpr (K32) {
+            kahan_pixel(ks, kc, T, g, dx, dy);
+            return;
+        }
         // FP64: every product SD_k T is exact, and where T takes integer values
         // (the integer pass, uniform target regions) the sums are exact, so
         // T = R gives an exactly zero update and near-flat subsets, whose
@@ -379,6 +404,11 @@ __device__ __forceinline__ bool subset_loop(const IcgnParams& P, int px, int py,
     // 2^24 and 2^48 scale the sums of T 2^-24 back exactly (FP64, far from
     // its range limits)
     const double sc = SCALED ? 0x1p24 : 1.0, sc2 = SCALED ? 0x1p48 : 1.0, hsc = SCALED ? 0x1p23 : 0.5;
+    if constexpr (K32) {
+        st = (double)ks[0] - (double)kc[0]; stt = (double)ks[1] - (double)kc[1];
+        a0 = (double)ks[2] - (double)kc[2]; a1 = (double)ks[3] - (double)kc[3]; a2 = (double)ks[4] - (double)kc[4];
+        b0 = (double)ks[5] - (double)kc[5]; b1 = (double)ks[6] - (double)kc[6]; b2 = (double)ks[7] - (double)kc[7];
+    }
     acc[0] = sc * st; acc[1] = sc2 * stt;
     acc[2] = hsc * a0; acc[3] = hsc * a1; acc[4] = hsc * a2;
     acc[5] = hsc * b0; acc[6] = hsc * b1; acc[7] = hsc * b2;
@@ -394,7 +424,7 @@ __device__ __forceinline__ bool subset_loop(const IcgnParams& P, int px, int py,
 // region where no per-pixel test can fire (the margin covers FP32 rounding
 // of the per-pixel warp, which is below 1e-3 px), the pass runs without the
 // per-pixel tests; the results are identical either way.
-template <int INTERP, bool INT, bool TABLE, bool RP>
+template <int INTERP, bool INT, bool TABLE, bool RP, bool K32 = false>
 __device__ __forceinline__ bool subset_pass(const IcgnParams& P, int px, int py, const float Wm[6], int lane,
                                             double acc[8], const StepTable* tab)
 {
@@ -407,13 +437,13 @@ __device__ __forceinline__ bool subset_pass(const IcgnParams& P, int px, int py,
                       fmaxf(fmaxf(x0, x1), fmaxf(x2, x3)) < (float)(P.sx0 + P.rw - 2) &&
                       fminf(fminf(y0, y1), fminf(y2, y3)) >= (float)(P.sy0 + 1) &&
                       fmaxf(fmaxf(y0, y1), fmaxf(y2, y3)) < (float)(P.sy0 + P.rh - 2);
-    if (safe) return subset_loop<INTERP, INT, false, TABLE, RP>(P, px, py, Wm, lane, acc, tab);
-    return subset_loop<INTERP, INT, true, TABLE, RP>(P, px, py, Wm, lane, acc, tab);
+    if (safe) return subset_loop<INTERP, INT, false, TABLE, RP, K32>(P, px, py, Wm, lane, acc, tab);
+    return subset_loop<INTERP, INT, true, TABLE, RP, K32>(P, px, py, Wm, lane, acc, tab);
 }
 
 // IC-GN for one POI by one wavefront.  poi is wave-uniform, so per-POI state
 // is read with scalar loads into SGPRs.
-template <int INTERP, bool TABLE, bool RP>
+template <int INTERP, bool TABLE, bool RP, bool K32 = false>
 __device__ __forceinline__ void icgn_poi_t(const IcgnParams& P, int poi, int lane, const StepTable* tab)
 {
     const int py = __builtin_amdgcn_readfirstlane(P.poi[2 * poi]);
@@ -437,8 +467,8 @@ __device__ __forceinline__ void icgn_poi_t(const IcgnParams& P, int poi, int lan
         for (int it = 0;; ++it) {
             if (it >= 1) iters = it + 1;                       // ++iNumIterations (:753)
             double acc[8];
-            const bool inside = it == 0 ? subset_pass<INTERP, true, TABLE, RP>(P, px, py, Wm, lane, acc, tab)
-                                        : subset_pass<INTERP, false, TABLE, RP>(P, px, py, Wm, lane, acc, tab);
+            const bool inside = it == 0 ? subset_pass<INTERP, true, TABLE, RP, K32>(P, px, py, Wm, lane, acc, tab)
+                                        : subset_pass<INTERP, false, TABLE, RP, K32>(P, px, py, Wm, lane, acc, tab);
             if (!inside) { flag = TWDIC_ICGN_OUT_OF_ROI; break; }         // :632-635, :787-790
 #pragma unroll
             for (int i = 0; i < 8; ++i) acc[i] = wave_sum(acc[i]);
@@ -532,7 +562,7 @@ __device__ __forceinline__ int xcd_block(int b, int nb)
     return x * q + min(x, r) + slot;
 }
 
-template <int INTERP, int MINW, bool TABLE, bool RP>
+template <int INTERP, int MINW, bool TABLE, bool RP, bool K32 = false>
 __global__ __launch_bounds__(256, MINW > 0 ? MINW : 1) void k_icgn(IcgnParams P)
 {
     const int poi = __builtin_amdgcn_readfirstlane(xcd_block(blockIdx.x, gridDim.x) * 4 + (threadIdx.x >> 6));
@@ -541,10 +571,10 @@ __global__ __launch_bounds__(256, MINW > 0 ? MINW : 1) void k_icgn(IcgnParams P)
         fill_step_table(tab, P);
         __syncthreads();
         if (poi >= P.n) return;
-        icgn_poi_t<INTERP, true, RP>(P, poi, threadIdx.x & 63, &tab);
+        icgn_poi_t<INTERP, true, RP, K32>(P, poi, threadIdx.x & 63, &tab);
     } else {
         if (poi >= P.n) return;
-        icgn_poi_t<INTERP, false, RP>(P, poi, threadIdx.x & 63, nullptr);
+        icgn_poi_t<INTERP, false, RP, K32>(P, poi, threadIdx.x & 63, nullptr);
     }
 }
 
@@ -661,6 +691,7 @@ static void launch_icgn_v(const IcgnParams& p, int variant, hipStream_t st)
 {
     const dim3 grid((p.n + 3) / 4);
     switch (variant) {
+    case 20: hipLaunchKernelGGL((k_icgn<INTERP, 5, TABLE, RP, true>), grid, dim3(256), 0, st, p); break;
     case 5: hipLaunchKernelGGL((k_icgn<INTERP, 5, TABLE, RP>), grid, dim3(256), 0, st, p); break;
     case 6: hipLaunchKernelGGL((k_icgn<INTERP, 6, TABLE, RP>), grid, dim3(256), 0, st, p); break;
     case 7: hipLaunchKernelGGL((k_icgn<INTERP, 7, TABLE, RP>), grid, dim3(256), 0, st, p); break;
