@@ -76,3 +76,41 @@ def test_sweep(seed):
     cap = max(2, int(0.002 * len(poi)), 3 * own_flips + 2)
     check_icgn(g, o, gtol=max(3e-6, 2 * own), max_flips=cap)
     check_icgn(g, o64, utol=5e-5, gtol=max(2e-6, 1.5 * own), max_flips=cap)
+
+
+@pytest.mark.parametrize("seed", range(4))
+def test_sweep_rectangular_subsets(seed):
+    """Subsets with Sx != Sy (the C ABI and IC-GN take the half sizes
+    separately; every other workload is square): fused interpolation from the
+    row polynomials on even seeds, the LUT on odd ones, against both oracles
+    with the bars above."""
+    rng = np.random.default_rng(5000 + seed)
+    w, h = int(rng.integers(160, 321)), int(rng.integers(160, 321))
+    sx, sy = int(rng.integers(8, 26)), int(rng.integers(8, 26))
+    if sx == sy:
+        sy = sx + 3
+    N = int(2 * rng.integers(8, 17))
+    d = api.affine(*rng.uniform(-2, 2, 2), *rng.uniform(-0.008, 0.008, 3), w / 2, h / 2)
+    ref, tar = api.synth_pair(w, h, int(0.015 * w * h), 6000 + seed, d, noise_sigma=2.0)
+    dmax = math.ceil(2 + 0.008 * max(w, h))
+    mx, my = max(N // 2, sx + dmax + 2) - sx, max(N // 2, sy + dmax + 2) - sy
+    gx, gy = int(rng.integers(4, 11)), int(rng.integers(4, 11))
+    interp = api.INTERP_FUSED if seed % 2 == 0 else api.INTERP_LUT
+    cfg = api.make_config(w, h, 1, 1, w - 2, h - 2, sx, sy, gx, gy, mx, my, N, interp=interp)
+    with api.Context(cfg) as ctx:
+        ctx.set_reference(ref)
+        ctx.set_target(tar)
+        poi = ctx.poi_grid().download()
+        f = run_fftcc(ctx, poi)
+        g = run_icgn(ctx, poi, f["u"], f["v"])
+    print(f"  {w}x{h} Sx={sx} Sy={sy} N={N} n={len(poi)} interp={interp}", flush=True)
+    roi = (1, 1, w - 2, h - 2)
+    o = oracle.icgn(ref, tar, roi, sx, sy, poi, f["u"], f["v"])
+    o64 = oracle.icgn(ref, tar, roi, sx, sy, poi, f["u"].astype(np.float64), f["v"].astype(np.float64),
+                      precision="f64")
+    both = (o["flags"] == oracle.SUCCESS) & (o["iters"] == o64["iters"])
+    own = float(np.max(np.abs(o["p6"][both][:, [1, 2, 4, 5]] - o64["p6"][both][:, [1, 2, 4, 5]]))) if both.any() else 0.0
+    own_flips = int(np.sum(o["iters"] != o64["iters"]))
+    cap = max(2, int(0.002 * len(poi)), 3 * own_flips + 2)
+    check_icgn(g, o, gtol=max(3e-6, 2 * own), max_flips=cap)
+    check_icgn(g, o64, utol=5e-5, gtol=max(2e-6, 1.5 * own), max_flips=cap)
