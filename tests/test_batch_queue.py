@@ -7,8 +7,8 @@ The host builds, per image pair i with n_i POIs (batch_api.cpp set_pairs):
   * fitems[c]: the same per FFT-CC window class c = 0 (N = 64), 1 (N = 32),
     2 (other N) (FFT-CC, persistent form, two items per workgroup grab);
   * mitems[f]: per matrix-core kernel form (FFT-CC, flat form);
-  * wg[t]: (i, k) for k = 0, 4, 8, ... of each pair, t = 0 when the subset fits
-    the step table (IC-GN, flat form, four POIs per workgroup).
+  * wg[t]: (i, k) for k = 0, 8, 16, ... of each pair, t = 0 when the subset fits
+    the step table (IC-GN, flat form, eight POIs per workgroup).
 and each persistent loop is
   ref state:  i = atomicAdd(c, 1)     per wavefront; exit when i >= total
   IC-GN:      b = atomicAdd(c, chunk) per wavefront; items [b, min(b + chunk, total));
@@ -28,6 +28,10 @@ import numpy as np
 import pytest
 
 from tests import cases
+
+CSRC = Path(__file__).resolve().parents[1] / "tw_amd" / "csrc"
+# POIs per IC-GN workgroup item, as the library defines it
+POIS_PER_WG = int(re.search(r"constexpr int kIcgnPoisPerWg = (\d+);", (CSRC / "twdic_internal.h").read_text()).group(1))
 
 
 def mma_form(N):
@@ -50,7 +54,7 @@ def work_lists(layout):
     wg = [[], []]
     for i, n in enumerate(counts):
         t = 0 if side[i] * side[i] <= 2048 and side[i] <= 48 else 1
-        wg[t].extend((i, k) for k in range(0, n, 4))
+        wg[t].extend((i, k) for k in range(0, n, POIS_PER_WG))
     wg = [np.array(w, np.int64).reshape(-1, 2) for w in wg]
     return counts, items, fitems, mitems, wg
 
@@ -76,11 +80,11 @@ def test_lists_name_only_existing_pois(c3):
     for lst in [items, *fitems, *mitems]:
         check_entries(lst, counts)
     for w in wg:
-        check_entries(w, counts, 4)
+        check_entries(w, counts, POIS_PER_WG)
     # every POI of every pair is in exactly one class / form list and in one
-    # four-POI workgroup item
+    # eight-POI workgroup item
     assert sum(len(f) for f in fitems) == len(items) and sum(len(m) for m in mitems) == len(items)
-    assert sum(len(w) for w in wg) == int(sum((n + 3) // 4 for n in counts))
+    assert sum(len(w) for w in wg) == int(sum((n + POIS_PER_WG - 1) // POIS_PER_WG for n in counts))
 
 
 def emulate(total, actors, grab, cover, rng, max_steps):
@@ -149,7 +153,7 @@ def test_counter_reset_covers_every_persistent_counter():
     clears and the counters each launch is given).  A counter left at its
     previous launch's end would make every grab fail at once: the launch would
     skip its items (no hang), so all of them must be cleared."""
-    src = (Path(__file__).resolve().parents[1] / "tw_amd" / "csrc" / "batch_api.cpp").read_text()
+    src = (CSRC / "batch_api.cpp").read_text()
     enum = re.search(r"enum \{ (CNT_STATE = 0[^}]*)\}", src).group(1)
     cnt = {k.strip(): int(v) for k, v in (e.split("=") for e in enum.split(","))}
     cleared = set()

@@ -401,14 +401,14 @@ twdic_status set_pairs_impl(twdic_batch* b, const twdic_pair* pairs, int npairs)
         }
     }
     if (total) TB_HIP(b, hipMemcpy(b->d_items, items.data(), sizeof(int2) * total, hipMemcpyHostToDevice));
-    // flat form: FFT-CC items per kernel form, IC-GN items of four POIs of one pair
+    // flat form: FFT-CC items per kernel form, IC-GN items of kIcgnPoisPerWg POIs of one pair
     std::vector<int2> mitems[4], wg[2];
     for (int i = 0; i < npairs; ++i) {
         const twdic_pair& p = b->pairs[i];
         auto& mi = mitems[mma_form(p.fft_n)];
         for (int k = 0; k < p.n; ++k) mi.push_back(make_int2(i, k));
         auto& w = wg[step_table_fits(2 * p.sx + 1, 2 * p.sy + 1) ? 0 : 1];
-        for (int k = 0; k < p.n; k += 4) w.push_back(make_int2(i, k));
+        for (int k = 0; k < p.n; k += kIcgnPoisPerWg) w.push_back(make_int2(i, k));
     }
     for (int f = 0; f < 4; ++f) {
         b->nm[f] = (int64_t)mitems[f].size();

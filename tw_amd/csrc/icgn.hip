@@ -1,5 +1,5 @@
 // icgn.hip -- first-order IC-GN sub-pixel registration for gfx950, one
-// wavefront per POI, four POIs per 256-thread workgroup.
+// 32-lane half of a wavefront per POI, eight POIs per 256-thread workgroup.
 //
 // Semantics follow ICGN2D_CPU::ICGN2D_Compute (TW_paDIC_ICGN2D_CPU.cpp:
 // 481-925): first pass at integer target pixels, later passes bicubic; flags
@@ -9,15 +9,21 @@
 // FP32 dp with the norm evaluated in FP64 as the reference's pow() does.
 //
 // Per-iteration work stays in registers:
-//   * each lane walks pixels lane, lane+64, ... of the (2Sy+1)x(2Sx+1) subset;
+//   * each lane of a half walks pixels l, l+32, ... (l = lane & 31) of the
+//     (2Sy+1)x(2Sx+1) subset;
 //   * one pass forms sum T, sum T^2 and the six sums SD_k T in FP64 (every
 //     FP32 product SD_k T is exact in FP64), so the residual vector is
 //       RHS_k = (R_s/T_s) (sum SD_k T - mean(T) A_k) - B_k
 //     with A_k = sum SD_k and B_k = sum SD_k (R - mean R) precomputed per
 //     reference (k_ref_state).  This equals the reference's three passes
 //     (:793-826) algebraically and is more accurate;
-//   * wavefront butterflies reduce the 8 sums (all lanes end bit-identical);
+//   * butterflies within the half reduce the 8 sums (its 32 lanes end
+//     bit-identical);
 //   * the 6x6 solve uses the per-POI FP64 Cholesky factor.
+// The per-POI work between passes (reductions, solve, warp update, loop
+// test) is the same instruction stream for the two POIs of a wavefront, so
+// it is issued once for both; a half whose POI has finished idles until the
+// other's has.
 #include "twdic_internal.h"
 
 #include <cfloat>
@@ -26,11 +32,20 @@ namespace twdic {
 
 namespace {
 
-__device__ __forceinline__ double wave_sum(double v)
+// Sum over the 32 lanes of a half-wavefront; every lane of the half ends with
+// the same bits.
+__device__ __forceinline__ double half_sum(double v)
 {
 #pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    for (int o = 16; o > 0; o >>= 1) v += __shfl_xor(v, o);
     return v;
+}
+
+// True when pred holds in any active lane of the caller's half.
+__device__ __forceinline__ bool half_any(bool pred, int lane)
+{
+    const unsigned long long b = __ballot(pred);
+    return (uint32_t)((lane & 32) ? (b >> 32) : b) != 0u;
 }
 
 // Warp coordinate exactly as the CPU path evaluates it (:618-619, :761-762):
@@ -269,8 +284,8 @@ __device__ __forceinline__ void kahan_pixel(float ks[8], float kc[8], float T, u
 // The subset's pixels in row-major order, p = l * Wsub + m: the exact float
 // offsets dx = m - Sx, dy = l - Sy and the byte offset 4 (l * rw + m) into
 // the gradient plane from the subset's corner.  Every POI of a launch has the
-// same subset and ROI width, so one table in LDS serves a workgroup's four
-// wavefronts, and a lane's step to its next pixel is an LDS read instead of
+// same subset and ROI width, so one table in LDS serves a workgroup's eight
+// POIs, and a lane's step to its next pixel is an LDS read instead of
 // float compares and selects.  The three arrays have the same 4-byte stride,
 // so one address register reaches all of them (the array offsets are
 // immediates of the LDS reads).
@@ -291,19 +306,22 @@ __device__ __forceinline__ void fill_step_table(StepTable& t, const IcgnParams& 
     }
 }
 
-// One pass over the subset at the current warp.  INT: first pass at integer
-// target pixels (:610-637); otherwise bicubic (:757-792).  Returns false when
-// a warped pixel leaves the ROI.  Accumulates sum T, sum T^2 and the six
-// sum SD_k T / 2 (the gradient plane stores 2 Rx, 2 Ry) in FP64.  With a step
-// table (tab != null) the lanes take pixels lane + 64 k from it: s / 64 full
-// rounds, then the s % 64 remaining pixels; without one they step their
-// offsets in registers.  Each lane visits the same pixels in the same order
-// either way, so the sums are identical.  RP: the fused bicubic passes
-// evaluate the target's row polynomials (P.rp) instead of the image bytes.
+// One pass over the subset at the current warp, by the 32 lanes of a half
+// (hl = lane & 31; px, py and the warp are the half's POI's).  INT: first pass
+// at integer target pixels (:610-637); otherwise bicubic (:757-792).  Returns
+// false when a warped pixel of the half's POI leaves the ROI.  Accumulates
+// sum T, sum T^2 and the six sum SD_k T / 2 (the gradient plane stores 2 Rx,
+// 2 Ry) in FP64.  With a step table (tab != null) the lanes take pixels
+// hl + 32 k from it: s / 32 full rounds, then the s % 32 remaining pixels;
+// without one they step their offsets in registers.  Each lane visits the
+// same pixels in the same order either way, so the sums are identical.  RP:
+// the fused bicubic passes evaluate the target's row polynomials (P.rp)
+// instead of the image bytes.
 template <int INTERP, bool INT, bool CHECK, bool TABLE, bool RP, bool K32 = false>
 __device__ __forceinline__ bool subset_loop(const IcgnParams& P, int px, int py, const float Wm[6], int lane,
                                             double acc[8], const StepTable* tab)
 {
+    const int hl = lane & 31;
     // the fused interpolation from the bytes returns T 2^-24 (interp_rows_s);
     // the sums are scaled back at the end, exactly
     constexpr bool SCALED = !INT && INTERP == TWDIC_INTERP_FUSED && !RP;
@@ -342,7 +360,7 @@ __device__ __forceinline__ bool subset_loop(const IcgnParams& P, int px, int py,
                 T = interp_lut(P.lut, (uint32_t)((iy - sy0) * rw + (ix - sx0)), tx, ty);
             }
         }
-        const uint32_t g = __builtin_amdgcn_raw_buffer_load_b32(g2r, (int)go, goff, 0);
+        const uint32_t g = __builtin_amdgcn_raw_buffer_load_b32(g2r, (int)go + goff, 0, 0);
         if constexpr (K32) {
             kahan_pixel(ks, kc, T, g, dx, dy);
             return;
@@ -366,38 +384,38 @@ __device__ __forceinline__ bool subset_loop(const IcgnParams& P, int px, int py,
     // byte offset of the subset's corner pixel in the gradient plane
     const int gcorner = 4 * ((py - Sy - sy0) * rw + (px - Sx - sx0));
     if constexpr (TABLE) {
-        const int full = s >> 6;
+        const int full = s >> 5;
 #pragma unroll 1
         for (int k = 0; k < full; ++k) {
-            const int p = lane + 64 * k;
+            const int p = hl + 32 * k;
             pixel(tab->dx[p], tab->dy[p], tab->go[p], gcorner);
         }
-        const int p = lane + 64 * full;
+        const int p = hl + 32 * full;
         if (p < s) {
             pixel(tab->dx[p], tab->dy[p], tab->go[p], gcorner);
         }
     } else {
-        // lane walks pixels p = lane, lane + 64, ...: column m = p % Wsub, row
+        // lane walks pixels p = hl, hl + 32, ...: column m = p % Wsub, row
         // l = p / Wsub, kept as the exact float offsets dx = m - Sx, dy = l - Sy
         // and the 32-bit byte offset into the gradient plane
-        const int q64 = 64 / Wsub, r64 = 64 - q64 * Wsub;
+        const int q32 = 32 / Wsub, r32 = 32 - q32 * Wsub;
         // recomputed per pass: kept live across the iteration loop, this start
         // state is what the register allocator spills
-        int lane_v = lane;
+        int lane_v = hl;
         asm volatile("" : "+v"(lane_v));
         const int l0 = lane_v / Wsub, m0 = lane_v - l0 * Wsub;
         float dx = (float)(m0 - Sx), dy = (float)(l0 - Sy);
         uint32_t go = (uint32_t)gcorner + 4u * (uint32_t)(l0 * rw + m0);
-        const float fSx = (float)Sx, fWsub = (float)Wsub, fr64 = (float)r64, fq64 = (float)q64;
-        const uint32_t gstep = 4u * (uint32_t)(q64 * rw + r64), gwrap = 4u * (uint32_t)(rw - Wsub);
+        const float fSx = (float)Sx, fWsub = (float)Wsub, fr32 = (float)r32, fq32 = (float)q32;
+        const uint32_t gstep = 4u * (uint32_t)(q32 * rw + r32), gwrap = 4u * (uint32_t)(rw - Wsub);
 #pragma unroll 1
-        for (int p = lane; p < s; p += 64) {
+        for (int p = hl; p < s; p += 32) {
             pixel(dx, dy, go, 0);
-            // next pixel of this lane: 64 further in row-major subset order
-            dx += fr64;
+            // next pixel of this lane: 32 further in row-major subset order
+            dx += fr32;
             const bool wrap = dx > fSx;
             dx = wrap ? dx - fWsub : dx;
-            dy += wrap ? fq64 + 1.f : fq64;
+            dy += wrap ? fq32 + 1.f : fq32;
             go += wrap ? gstep + gwrap : gstep;
         }
     }
@@ -412,7 +430,7 @@ __device__ __forceinline__ bool subset_loop(const IcgnParams& P, int px, int py,
     acc[0] = sc * st; acc[1] = sc2 * stt;
     acc[2] = hsc * a0; acc[3] = hsc * a1; acc[4] = hsc * a2;
     acc[5] = hsc * b0; acc[6] = hsc * b1; acc[7] = hsc * b2;
-    return CHECK ? !__any(out) : true;
+    return CHECK ? !half_any(out, lane) : true;
 }
 
 // One pass over the subset at the current warp.  INT: first pass at integer
@@ -437,17 +455,23 @@ __device__ __forceinline__ bool subset_pass(const IcgnParams& P, int px, int py,
                       fmaxf(fmaxf(x0, x1), fmaxf(x2, x3)) < (float)(P.sx0 + P.rw - 2) &&
                       fminf(fminf(y0, y1), fminf(y2, y3)) >= (float)(P.sy0 + 1) &&
                       fmaxf(fmaxf(y0, y1), fmaxf(y2, y3)) < (float)(P.sy0 + P.rh - 2);
-    if (safe) return subset_loop<INTERP, INT, false, TABLE, RP, K32>(P, px, py, Wm, lane, acc, tab);
+    // both halves take the same loop: the per-pixel tests run unless every
+    // active lane's POI is safe
+    if (__all(safe)) return subset_loop<INTERP, INT, false, TABLE, RP, K32>(P, px, py, Wm, lane, acc, tab);
     return subset_loop<INTERP, INT, true, TABLE, RP, K32>(P, px, py, Wm, lane, acc, tab);
 }
 
-// IC-GN for one POI by one wavefront.  poi is wave-uniform, so per-POI state
-// is read with scalar loads into SGPRs.
+// IC-GN for one POI by one half of a wavefront (poi is the half's; the other
+// half may run another POI of the same launch or none).  rs: the POI's
+// reference state, copied to LDS by the caller (its first kRefStateUsed
+// doubles).  Every lane of the half holds the same per-POI values, so the two
+// halves' solves and warp updates issue as one instruction stream.
 template <int INTERP, bool TABLE, bool RP, bool K32 = false>
-__device__ __forceinline__ void icgn_poi_t(const IcgnParams& P, int poi, int lane, const StepTable* tab)
+__device__ __forceinline__ void icgn_poi_t(const IcgnParams& P, int poi, int lane, const StepTable* tab,
+                                           const double* rs)
 {
-    const int py = __builtin_amdgcn_readfirstlane(P.poi[2 * poi]);
-    const int px = __builtin_amdgcn_readfirstlane(P.poi[2 * poi + 1]);
+    const int py = P.poi[2 * poi];
+    const int px = P.poi[2 * poi + 1];
     const int Sx = P.Sx, Sy = P.Sy;
     const double dn = (double)P.s;
 
@@ -457,8 +481,6 @@ __device__ __forceinline__ void icgn_poi_t(const IcgnParams& P, int poi, int lan
     int flag = TWDIC_ICGN_SUCCESS;
     float dpn = __builtin_nanf("");
 
-    // the state block is read-only here: constant address space -> scalar loads
-    const TW_CONST_AS double* rs = (const TW_CONST_AS double*)(P.refstate + (size_t)poi * kRefStateStride);
     const int refflag = (int)rs[RS_FLAG];
     if (refflag == TWDIC_ICGN_DARK_SUBSET || refflag == TWDIC_ICGN_OUT_OF_ROI) {
         flag = refflag;
@@ -471,7 +493,7 @@ __device__ __forceinline__ void icgn_poi_t(const IcgnParams& P, int poi, int lan
                                         : subset_pass<INTERP, false, TABLE, RP, K32>(P, px, py, Wm, lane, acc, tab);
             if (!inside) { flag = TWDIC_ICGN_OUT_OF_ROI; break; }         // :632-635, :787-790
 #pragma unroll
-            for (int i = 0; i < 8; ++i) acc[i] = wave_sum(acc[i]);
+            for (int i = 0; i < 8; ++i) acc[i] = half_sum(acc[i]);
             double meanT, normT;
             mean_spread(acc[0], acc[1], dn, meanT, normT);
             if (fabsf((float)meanT) <= FLT_EPSILON) { flag = TWDIC_ICGN_DARK_SUBSET; break; }   // :638-640
@@ -508,20 +530,13 @@ __device__ __forceinline__ void icgn_poi_t(const IcgnParams& P, int poi, int lan
             const float den = warp_den(d);
             if (fabsf(den) <= FLT_EPSILON) { flag = TWDIC_ICGN_SINGULAR_WARP; break; }        // :875-877
             warp_update(Pp, d, den, Wm);
-            // every lane holds the same warp (the butterflies leave the sums
-            // bit-identical in all lanes): keep it in SGPRs for the next pass
-#pragma unroll
-            for (int i = 0; i < 6; ++i) {
-                Wm[i] = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(Wm[i])));
-                Pp[i] = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(Pp[i])));
-            }
             const double nrm = conv_norm(d, Sx, Sy);
             dpn = (float)nrm;
             if (it == 0) iters = 1;                                    // :749
             if (!(iters < P.max_iter && nrm >= (double)P.delta_p)) break;
         }
     }
-    if (lane == 0) {
+    if ((lane & 31) == 0) {
         P.u[poi] = Pp[0];
         P.v[poi] = Pp[3];
         if (P.p6) {
@@ -538,12 +553,24 @@ __device__ __forceinline__ void icgn_poi_t(const IcgnParams& P, int poi, int lan
 // them (P.rp, the same for every wave of a launch), else the image bytes;
 // both evaluate the same Catmull-Rom spline.
 template <int INTERP, bool TABLE>
-__device__ __forceinline__ void icgn_poi(const IcgnParams& P, int poi, int lane, const StepTable* tab)
+__device__ __forceinline__ void icgn_poi(const IcgnParams& P, int poi, int lane, const StepTable* tab,
+                                         const double* rs)
 {
     if (INTERP == TWDIC_INTERP_FUSED && P.rp != nullptr)
-        icgn_poi_t<INTERP, TABLE, INTERP == TWDIC_INTERP_FUSED>(P, poi, lane, tab);
+        icgn_poi_t<INTERP, TABLE, INTERP == TWDIC_INTERP_FUSED>(P, poi, lane, tab, rs);
     else
-        icgn_poi_t<INTERP, TABLE, false>(P, poi, lane, tab);
+        icgn_poi_t<INTERP, TABLE, false>(P, poi, lane, tab, rs);
+}
+
+// The reference state of the POIs first .. first + count - 1 (those below n)
+// into slots 0 .. count - 1 of an LDS array, by the threads t0, t0 + tstride, ...
+__device__ __forceinline__ void copy_ref_state(double (*slots)[kRefStateUsed], const double* __restrict__ state,
+                                               int first, int count, int n, int t0, int tstride)
+{
+    for (int i = t0; i < count * kRefStateUsed; i += tstride) {
+        const int slot = i / kRefStateUsed, j = i - slot * kRefStateUsed;
+        if (first + slot < n) slots[slot][j] = state[(size_t)(first + slot) * kRefStateStride + j];
+    }
 }
 
 // The single-context kernel takes the choice at launch instead (RP: P.rp is
@@ -553,9 +580,10 @@ __device__ __forceinline__ void icgn_poi(const IcgnParams& P, int poi, int lane,
 // 0 leaves the choice to the compiler.
 // TABLE: the subset has at most kStepTable pixels; the workgroup builds the
 // step table in LDS first.
-// One wavefront per POI, four per workgroup; consecutive workgroups go to the
-// 8 XCDs round-robin, so block b is remapped to take POIs from XCD b % 8's
-// contiguous share and each XCD's L2 holds one band of the image.
+// Half a wavefront per POI, eight per workgroup (POI 8 b' + t / 32 for thread
+// t); consecutive workgroups go to the 8 XCDs round-robin, so block b is
+// remapped to b' = the next block of XCD b % 8's contiguous share and each
+// XCD's L2 holds one band of the image.
 __device__ __forceinline__ int xcd_block(int b, int nb)
 {
     const int x = b & 7, slot = b >> 3, q = nb >> 3, r = nb & 7;
@@ -565,22 +593,26 @@ __device__ __forceinline__ int xcd_block(int b, int nb)
 template <int INTERP, int MINW, bool TABLE, bool RP, bool K32 = false>
 __global__ __launch_bounds__(256, MINW > 0 ? MINW : 1) void k_icgn(IcgnParams P)
 {
-    const int poi = __builtin_amdgcn_readfirstlane(xcd_block(blockIdx.x, gridDim.x) * 4 + (threadIdx.x >> 6));
+    __shared__ double rs[kIcgnPoisPerWg][kRefStateUsed];
+    const int first = xcd_block(blockIdx.x, gridDim.x) * kIcgnPoisPerWg;
+    const int slot = threadIdx.x >> 5, poi = first + slot;
+    copy_ref_state(rs, P.refstate, first, kIcgnPoisPerWg, P.n, threadIdx.x, 256);
     if constexpr (TABLE) {
         __shared__ StepTable tab;
         fill_step_table(tab, P);
         __syncthreads();
         if (poi >= P.n) return;
-        icgn_poi_t<INTERP, true, RP, K32>(P, poi, threadIdx.x & 63, &tab);
+        icgn_poi_t<INTERP, true, RP, K32>(P, poi, threadIdx.x & 63, &tab, rs[slot]);
     } else {
+        __syncthreads();
         if (poi >= P.n) return;
-        icgn_poi_t<INTERP, false, RP, K32>(P, poi, threadIdx.x & 63, nullptr);
+        icgn_poi_t<INTERP, false, RP, K32>(P, poi, threadIdx.x & 63, nullptr, rs[slot]);
     }
 }
 
 // Batched form, one launch over many image pairs: workgroup item w =
-// (pair, first POI) holds up to four POIs of one pair, so the workgroup's step
-// table is that pair's.  XCD: items in contiguous per-XCD ranges as in
+// (pair, first POI) holds up to eight POIs of one pair, so the workgroup's
+// step table is that pair's.  XCD: items in contiguous per-XCD ranges as in
 // k_icgn (one band of the work per L2); otherwise dealt out in block order,
 // which balances the XCDs when the work per item differs from pair to pair.
 // ALL_RP: every pair has its row-polynomial plane (the usual case), so only
@@ -592,22 +624,26 @@ __global__ __launch_bounds__(256, MINW > 0 ? MINW : 1) void k_batch_icgn_wg(cons
 {
     const int2 it = wg[XCD ? xcd_block(blockIdx.x, gridDim.x) : (int)blockIdx.x];
     const IcgnParams P = const_pairs(pairs)[__builtin_amdgcn_readfirstlane(it.x)].icgn;
-    const int poi = __builtin_amdgcn_readfirstlane(it.y + (int)(threadIdx.x >> 6));
+    __shared__ double rs[kIcgnPoisPerWg][kRefStateUsed];
+    const int first = __builtin_amdgcn_readfirstlane(it.y);
+    const int slot = threadIdx.x >> 5, poi = first + slot;
+    copy_ref_state(rs, P.refstate, first, kIcgnPoisPerWg, P.n, threadIdx.x, 256);
     if constexpr (TABLE) {
         __shared__ StepTable tab;
         fill_step_table(tab, P);
         __syncthreads();
         if (poi >= P.n) return;
         if constexpr (ALL_RP)
-            icgn_poi_t<INTERP, true, true>(P, poi, threadIdx.x & 63, &tab);
+            icgn_poi_t<INTERP, true, true>(P, poi, threadIdx.x & 63, &tab, rs[slot]);
         else
-            icgn_poi<INTERP, true>(P, poi, threadIdx.x & 63, &tab);
+            icgn_poi<INTERP, true>(P, poi, threadIdx.x & 63, &tab, rs[slot]);
     } else {
+        __syncthreads();
         if (poi >= P.n) return;
         if constexpr (ALL_RP)
-            icgn_poi_t<INTERP, false, true>(P, poi, threadIdx.x & 63, nullptr);
+            icgn_poi_t<INTERP, false, true>(P, poi, threadIdx.x & 63, nullptr, rs[slot]);
         else
-            icgn_poi<INTERP, false>(P, poi, threadIdx.x & 63, nullptr);
+            icgn_poi<INTERP, false>(P, poi, threadIdx.x & 63, nullptr, rs[slot]);
     }
 }
 
@@ -650,11 +686,13 @@ hipError_t launch_batch_icgn_wg(const PairDesc* pairs, const int2* wg, int count
 // Persistent form: every wavefront takes `chunk` consecutive work items
 // (image pair, POI) from a device counter until the list is exhausted; each
 // item's geometry and buffers come from its pair's descriptor.  Every wave
-// exits once the counter passes `total`.
+// exits once the counter passes `total`.  Consecutive items may belong to
+// different pairs, so each item runs on the wavefront's lower half alone.
 __global__ __launch_bounds__(256) void k_batch_icgn(const PairDesc* __restrict__ pairs, const int2* __restrict__ items,
                                                     int total, int* counter, int chunk)
 {
-    const int lane = threadIdx.x & 63;
+    __shared__ double rs[4][1][kRefStateUsed];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     for (;;) {
         int base = 0;
         if (lane == 0) base = atomicAdd(counter, chunk);
@@ -665,7 +703,12 @@ __global__ __launch_bounds__(256) void k_batch_icgn(const PairDesc* __restrict__
             const int2 it = items[i];
             const int pair = __builtin_amdgcn_readfirstlane(it.x), poi = __builtin_amdgcn_readfirstlane(it.y);
             const IcgnParams P = const_pairs(pairs)[pair].icgn;
-            icgn_poi<TWDIC_INTERP_FUSED, false>(P, poi, lane, nullptr);
+            // the wave's LDS slot: written and read by this wavefront only
+            __builtin_amdgcn_wave_barrier();
+            copy_ref_state(rs[wave], P.refstate, poi, 1, P.n, lane, 64);
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            if (lane < 32) icgn_poi<TWDIC_INTERP_FUSED, false>(P, poi, lane, nullptr, rs[wave][0]);
         }
     }
 }
@@ -689,7 +732,7 @@ hipError_t launch_batch_icgn(const PairDesc* pairs, const int2* items, int total
 template <int INTERP, bool TABLE, bool RP>
 static void launch_icgn_v(const IcgnParams& p, int variant, hipStream_t st)
 {
-    const dim3 grid((p.n + 3) / 4);
+    const dim3 grid((p.n + kIcgnPoisPerWg - 1) / kIcgnPoisPerWg);
     switch (variant) {
     case 20: hipLaunchKernelGGL((k_icgn<INTERP, 5, TABLE, RP, true>), grid, dim3(256), 0, st, p); break;
     case 5: hipLaunchKernelGGL((k_icgn<INTERP, 5, TABLE, RP>), grid, dim3(256), 0, st, p); break;

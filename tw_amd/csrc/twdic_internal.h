@@ -26,6 +26,11 @@ constexpr int kRefStateStride = 48;   // doubles of per-reference state per POI 
 //   [35]     mean R
 //   [36..41] 1 / L_ii (reciprocal pivots of the factor)
 enum { RS_L = 0, RS_A = 21, RS_B = 27, RS_NORM = 33, RS_FLAG = 34, RS_MEAN = 35, RS_RDIAG = 36 };
+constexpr int kRefStateUsed = 42;     // the doubles above that the iteration kernels read
+
+// IC-GN runs one POI per 32-lane half-wavefront, eight POIs per 256-thread
+// workgroup; the batched form's workgroup items hold this many POIs of a pair.
+constexpr int kIcgnPoisPerWg = 8;
 
 // Mean and L2 spread of n samples from their exact-or-FP64 sums s1 = sum x,
 // s2 = sum x^2.  Shared by the reference-state and iteration kernels so that
@@ -133,7 +138,7 @@ hipError_t launch_fftcc(const FftccParams& p, int variant, hipStream_t st);
 int mma_form(int N);
 hipError_t launch_batch_fftcc_mma(const PairDesc* pairs, const int2* items, int count, int form, hipStream_t st);
 // batched form: IC-GN (fused interpolation) over workgroup items (pair, first
-// POI), four POIs of one pair per workgroup; `table` when every listed pair's
+// POI), kIcgnPoisPerWg POIs of one pair per workgroup; `table` when every listed pair's
 // subset fits the step table (step_table_fits).
 bool step_table_fits(int wsub, int hsub);
 // The row-polynomial plane of an ROI is addressed with 32-bit byte offsets
