@@ -32,20 +32,20 @@ namespace twdic {
 
 namespace {
 
-// Sum over the 32 lanes of a half-wavefront; every lane of the half ends with
-// the same bits.
+// Sum over the kIcgnLanes lanes of a POI's group (a half-wavefront); every
+// lane of the group ends with the same bits.
 __device__ __forceinline__ double half_sum(double v)
 {
 #pragma unroll
-    for (int o = 16; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    for (int o = kIcgnLanes / 2; o > 0; o >>= 1) v += __shfl_xor(v, o);
     return v;
 }
 
-// True when pred holds in any active lane of the caller's half.
+// True when pred holds in any active lane of the caller's group.
 __device__ __forceinline__ bool half_any(bool pred, int lane)
 {
-    const unsigned long long b = __ballot(pred);
-    return (uint32_t)((lane & 32) ? (b >> 32) : b) != 0u;
+    const unsigned long long b = __ballot(pred) >> (lane & (64 - kIcgnLanes));
+    return (b & ((1ull << kIcgnLanes) - 1)) != 0ull;
 }
 
 // Warp coordinate exactly as the CPU path evaluates it (:618-619, :761-762):
@@ -221,11 +221,28 @@ __device__ __forceinline__ float interp_row_poly(const RowPolyBuf& B, int ix, in
     return fmaf(0.5f * ty, fmaf(fmaf(d3, ty, d2), ty, d1), R1);
 }
 
-// W(p) o W(dp)^-1 and p from W: TW_paDIC_ICGN2D_CPU.cpp:713-745 / :875-907,
-// in FP32 with every operation rounded separately (no contraction).
-__device__ __forceinline__ void warp_update(float P[6], const float d[6], float den, float W[6])
+// p from W, as the update sets it (TW_paDIC_ICGN2D_CPU.cpp:902-907): p0 = W02,
+// p1 = W00 - 1, p2 = W01, p3 = W12, p4 = W10, p5 = W11 - 1.  For the initial
+// W = [[1 + 0, 0, u], [0, 1 + 0, v]] (:605-607) this gives back (u, 0, 0, v,
+// 0, 0), so p need not be kept beside W.
+__device__ __forceinline__ void params_of(const float W[6], float P[6])
 {
 #pragma clang fp contract(off)
+    P[0] = W[2];
+    P[1] = W[0] - 1;
+    P[2] = W[1];
+    P[3] = W[5];
+    P[4] = W[3];
+    P[5] = W[4] - 1;
+}
+
+// W <- W(p) o W(dp)^-1 with p = params_of(W): TW_paDIC_ICGN2D_CPU.cpp:713-745,
+// in FP32 with every operation rounded separately (no contraction).
+__device__ __forceinline__ void warp_update(const float d[6], float den, float W[6])
+{
+#pragma clang fp contract(off)
+    float P[6];
+    params_of(W, P);
     float w00 = ((1 + P[1]) * (1 + d[5]) - P[2] * d[4]) / den;
     float w01 = (P[2] * (1 + d[1]) - (1 + P[1]) * d[2]) / den;
     float w02 = P[0] + (P[2] * (d[0] * d[4] - d[3] - d[3] * d[1]) - (1 + P[1]) * (d[0] * d[5] + d[0] - d[2] * d[3])) / den;
@@ -233,12 +250,6 @@ __device__ __forceinline__ void warp_update(float P[6], const float d[6], float 
     float w11 = ((1 + P[5]) * (1 + d[1]) - P[4] * d[2]) / den;
     float w12 = P[3] + ((1 + P[5]) * (d[0] * d[4] - d[3] - d[3] * d[1]) - P[4] * (d[0] * d[5] + d[0] - d[2] * d[3])) / den;
     W[0] = w00; W[1] = w01; W[2] = w02; W[3] = w10; W[4] = w11; W[5] = w12;
-    P[0] = w02;
-    P[1] = w00 - 1;
-    P[2] = w01;
-    P[3] = w12;
-    P[4] = w10;
-    P[5] = w11 - 1;
 }
 
 __device__ __forceinline__ float warp_den(const float d[6])
@@ -321,7 +332,7 @@ template <int INTERP, bool INT, bool CHECK, bool TABLE, bool RP, bool K32 = fals
 __device__ __forceinline__ bool subset_loop(const IcgnParams& P, int px, int py, const float Wm[6], int lane,
                                             double acc[8], const StepTable* tab)
 {
-    const int hl = lane & 31;
+    const int hl = lane & (kIcgnLanes - 1);
     // the fused interpolation from the bytes returns T 2^-24 (interp_rows_s);
     // the sums are scaled back at the end, exactly
     constexpr bool SCALED = !INT && INTERP == TWDIC_INTERP_FUSED && !RP;
@@ -384,21 +395,22 @@ __device__ __forceinline__ bool subset_loop(const IcgnParams& P, int px, int py,
     // byte offset of the subset's corner pixel in the gradient plane
     const int gcorner = 4 * ((py - Sy - sy0) * rw + (px - Sx - sx0));
     if constexpr (TABLE) {
-        const int full = s >> 5;
+        const int full = s / kIcgnLanes;
 #pragma unroll 1
         for (int k = 0; k < full; ++k) {
-            const int p = hl + 32 * k;
+            const int p = hl + kIcgnLanes * k;
             pixel(tab->dx[p], tab->dy[p], tab->go[p], gcorner);
         }
-        const int p = hl + 32 * full;
+        const int p = hl + kIcgnLanes * full;
         if (p < s) {
             pixel(tab->dx[p], tab->dy[p], tab->go[p], gcorner);
         }
     } else {
-        // lane walks pixels p = hl, hl + 32, ...: column m = p % Wsub, row
-        // l = p / Wsub, kept as the exact float offsets dx = m - Sx, dy = l - Sy
-        // and the 32-bit byte offset into the gradient plane
-        const int q32 = 32 / Wsub, r32 = 32 - q32 * Wsub;
+        // lane walks pixels p = hl, hl + L, ... (L = kIcgnLanes): column
+        // m = p % Wsub, row l = p / Wsub, kept as the exact float offsets
+        // dx = m - Sx, dy = l - Sy and the 32-bit byte offset into the
+        // gradient plane
+        const int qL = kIcgnLanes / Wsub, rL = kIcgnLanes - qL * Wsub;
         // recomputed per pass: kept live across the iteration loop, this start
         // state is what the register allocator spills
         int lane_v = hl;
@@ -406,16 +418,16 @@ __device__ __forceinline__ bool subset_loop(const IcgnParams& P, int px, int py,
         const int l0 = lane_v / Wsub, m0 = lane_v - l0 * Wsub;
         float dx = (float)(m0 - Sx), dy = (float)(l0 - Sy);
         uint32_t go = (uint32_t)gcorner + 4u * (uint32_t)(l0 * rw + m0);
-        const float fSx = (float)Sx, fWsub = (float)Wsub, fr32 = (float)r32, fq32 = (float)q32;
-        const uint32_t gstep = 4u * (uint32_t)(q32 * rw + r32), gwrap = 4u * (uint32_t)(rw - Wsub);
+        const float fSx = (float)Sx, fWsub = (float)Wsub, frL = (float)rL, fqL = (float)qL;
+        const uint32_t gstep = 4u * (uint32_t)(qL * rw + rL), gwrap = 4u * (uint32_t)(rw - Wsub);
 #pragma unroll 1
-        for (int p = hl; p < s; p += 32) {
+        for (int p = hl; p < s; p += kIcgnLanes) {
             pixel(dx, dy, go, 0);
-            // next pixel of this lane: 32 further in row-major subset order
-            dx += fr32;
+            // next pixel of this lane: L further in row-major subset order
+            dx += frL;
             const bool wrap = dx > fSx;
             dx = wrap ? dx - fWsub : dx;
-            dy += wrap ? fq32 + 1.f : fq32;
+            dy += wrap ? fqL + 1.f : fqL;
             go += wrap ? gstep + gwrap : gstep;
         }
     }
@@ -475,8 +487,8 @@ __device__ __forceinline__ void icgn_poi_t(const IcgnParams& P, int poi, int lan
     const int Sx = P.Sx, Sy = P.Sy;
     const double dn = (double)P.s;
 
-    float Pp[6] = {P.u[poi], 0.f, 0.f, P.v[poi], 0.f, 0.f};
-    float Wm[6] = {1.f + Pp[1], Pp[2], Pp[0], Pp[4], 1.f + Pp[5], Pp[3]};  // :605-607
+    const float u0 = P.u[poi], v0 = P.v[poi];
+    float Wm[6] = {1.f + 0.f, 0.f, u0, 0.f, 1.f + 0.f, v0};  // :605-607 with p = (u, 0, 0, v, 0, 0)
     int iters = 0;                 // 0: leave the caller's value untouched
     int flag = TWDIC_ICGN_SUCCESS;
     float dpn = __builtin_nanf("");
@@ -485,7 +497,6 @@ __device__ __forceinline__ void icgn_poi_t(const IcgnParams& P, int poi, int lan
     if (refflag == TWDIC_ICGN_DARK_SUBSET || refflag == TWDIC_ICGN_OUT_OF_ROI) {
         flag = refflag;
     } else {
-        const double normR = rs[RS_NORM], meanR = rs[RS_MEAN];
         for (int it = 0;; ++it) {
             if (it >= 1) iters = it + 1;                       // ++iNumIterations (:753)
             double acc[8];
@@ -502,7 +513,12 @@ __device__ __forceinline__ void icgn_poi_t(const IcgnParams& P, int poi, int lan
                 flag = TWDIC_ICGN_SINGULAR_HESSIAN;
                 break;
             }
-            // RHS_k = sum SD_k ((R_s/T_s) (T - mean T) - (R - mean R))   (:657-667)
+            // RHS_k = sum SD_k ((R_s/T_s) (T - mean T) - (R - mean R))   (:657-667);
+            // R_s and mean R are read here, not held across the pass, which
+            // keeps the pass within the register budget of five waves per SIMD
+            int at = 0;
+            asm volatile("" : "+v"(at));
+            const double normR = rs[RS_NORM + at], meanR = rs[RS_MEAN + at];
             const double nor = normR / normT;
             double r[6];
 #pragma unroll
@@ -529,14 +545,16 @@ __device__ __forceinline__ void icgn_poi_t(const IcgnParams& P, int poi, int lan
             for (int i = 0; i < 6; ++i) d[i] = (float)r[i];
             const float den = warp_den(d);
             if (fabsf(den) <= FLT_EPSILON) { flag = TWDIC_ICGN_SINGULAR_WARP; break; }        // :875-877
-            warp_update(Pp, d, den, Wm);
+            warp_update(d, den, Wm);
             const double nrm = conv_norm(d, Sx, Sy);
             dpn = (float)nrm;
             if (it == 0) iters = 1;                                    // :749
-            if (!(iters < P.max_iter && nrm >= (double)P.delta_p)) break;
+            if (!(iters < P.max_iter && nrm >= P.delta_p)) break;
         }
     }
-    if ((lane & 31) == 0) {
+    if ((lane & (kIcgnLanes - 1)) == 0) {
+        float Pp[6];
+        params_of(Wm, Pp);
         P.u[poi] = Pp[0];
         P.v[poi] = Pp[3];
         if (P.p6) {
@@ -595,7 +613,7 @@ __global__ __launch_bounds__(256, MINW > 0 ? MINW : 1) void k_icgn(IcgnParams P)
 {
     __shared__ double rs[kIcgnPoisPerWg][kRefStateUsed];
     const int first = xcd_block(blockIdx.x, gridDim.x) * kIcgnPoisPerWg;
-    const int slot = threadIdx.x >> 5, poi = first + slot;
+    const int slot = threadIdx.x / kIcgnLanes, poi = first + slot;
     copy_ref_state(rs, P.refstate, first, kIcgnPoisPerWg, P.n, threadIdx.x, 256);
     if constexpr (TABLE) {
         __shared__ StepTable tab;
@@ -626,7 +644,7 @@ __global__ __launch_bounds__(256, MINW > 0 ? MINW : 1) void k_batch_icgn_wg(cons
     const IcgnParams P = const_pairs(pairs)[__builtin_amdgcn_readfirstlane(it.x)].icgn;
     __shared__ double rs[kIcgnPoisPerWg][kRefStateUsed];
     const int first = __builtin_amdgcn_readfirstlane(it.y);
-    const int slot = threadIdx.x >> 5, poi = first + slot;
+    const int slot = threadIdx.x / kIcgnLanes, poi = first + slot;
     copy_ref_state(rs, P.refstate, first, kIcgnPoisPerWg, P.n, threadIdx.x, 256);
     if constexpr (TABLE) {
         __shared__ StepTable tab;
@@ -708,7 +726,7 @@ __global__ __launch_bounds__(256) void k_batch_icgn(const PairDesc* __restrict__
             copy_ref_state(rs[wave], P.refstate, poi, 1, P.n, lane, 64);
             __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
             __builtin_amdgcn_wave_barrier();
-            if (lane < 32) icgn_poi<TWDIC_INTERP_FUSED, false>(P, poi, lane, nullptr, rs[wave][0]);
+            if (lane < kIcgnLanes) icgn_poi<TWDIC_INTERP_FUSED, false>(P, poi, lane, nullptr, rs[wave][0]);
         }
     }
 }

@@ -28,9 +28,11 @@ constexpr int kRefStateStride = 48;   // doubles of per-reference state per POI 
 enum { RS_L = 0, RS_A = 21, RS_B = 27, RS_NORM = 33, RS_FLAG = 34, RS_MEAN = 35, RS_RDIAG = 36 };
 constexpr int kRefStateUsed = 42;     // the doubles above that the iteration kernels read
 
-// IC-GN runs one POI per 32-lane half-wavefront, eight POIs per 256-thread
-// workgroup; the batched form's workgroup items hold this many POIs of a pair.
-constexpr int kIcgnPoisPerWg = 8;
+// IC-GN runs one POI per group of kIcgnLanes lanes of a wavefront (a half),
+// eight POIs per 256-thread workgroup; the batched form's workgroup items hold
+// this many POIs of a pair.
+constexpr int kIcgnLanes = 32;
+constexpr int kIcgnPoisPerWg = 256 / kIcgnLanes;
 
 // Mean and L2 spread of n samples from their exact-or-FP64 sums s1 = sum x,
 // s2 = sum x^2.  Shared by the reference-state and iteration kernels so that
@@ -71,7 +73,7 @@ struct IcgnParams {
     int sx0, sy0, rw, rh;     // ROI
     int Sx, Sy, Wsub, Hsub, s;
     int max_iter;
-    float delta_p;
+    double delta_p;           // the caller's float, widened: the loop test compares in FP64
     const uint32_t* g2;       // reference gradients (2Rx, 2Ry) as int16 pairs, ROI-relative
     const float* lut;         // target LUT [rh][rw][16]
     const uint4* rp;          // target row polynomials [rh + 1][rw] (fused mode; null: from the image bytes)
