@@ -30,8 +30,11 @@ import pytest
 from tests import cases
 
 CSRC = Path(__file__).resolve().parents[1] / "tw_amd" / "csrc"
-# POIs per IC-GN workgroup item, as the library defines it
-POIS_PER_WG = int(re.search(r"constexpr int kIcgnPoisPerWg = (\d+);", (CSRC / "twdic_internal.h").read_text()).group(1))
+# POIs per IC-GN workgroup item, as the library defines it: one per group of
+# kIcgnLanes lanes of a 256-thread workgroup
+_HDR = (CSRC / "twdic_internal.h").read_text()
+assert "constexpr int kIcgnPoisPerWg = 256 / kIcgnLanes;" in _HDR
+POIS_PER_WG = 256 // int(re.search(r"constexpr int kIcgnLanes = (\d+);", _HDR).group(1))
 
 
 def mma_form(N):
