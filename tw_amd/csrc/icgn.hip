@@ -347,8 +347,18 @@ __device__ __forceinline__ bool subset_loop(const IcgnParams& P, int px, int py,
     float ks[8] = {}, kc[8] = {};
     bool out = false;
     auto pixel = [&](float dx, float dy, uint32_t go, int goff) {
-        const float wx = warp_coord(fpx, Wm[0], Wm[1], Wm[2], dx, dy);
-        const float wy = warp_coord(fpy, Wm[3], Wm[4], Wm[5], dx, dy);
+        float wx, wy;
+        if constexpr (INT) {
+            // the first pass's warp is W = [[1, 0, u], [0, 1, v]] exactly, where
+            // ((POI + 1 dx) + 0 dy) + u = (POI + dx) + u bit for bit (POI + dx
+            // is a non-zero integer, so adding the zero product changes nothing)
+#pragma clang fp contract(off)
+            wx = (fpx + dx) + Wm[2];
+            wy = (fpy + dy) + Wm[5];
+        } else {
+            wx = warp_coord(fpx, Wm[0], Wm[1], Wm[2], dx, dy);
+            wy = warp_coord(fpy, Wm[3], Wm[4], Wm[5], dx, dy);
+        }
         const int ix = (int)wx, iy = (int)wy;          // truncation, as int()
         float T = 0.f;
         bool in = true, interior = true;
