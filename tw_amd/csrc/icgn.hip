@@ -32,13 +32,27 @@ namespace twdic {
 
 namespace {
 
+template <int O>
+__device__ __forceinline__ double add_lane_xor(double v)
+{
+    // the value of lane ^ O: ds_swizzle's bit mode (and-mask 0x1f, or-mask 0,
+    // xor-mask O) within 32-lane groups, one dword at a time
+    constexpr int pattern = 0x1f | (O << 10);
+    const int lo = __builtin_amdgcn_ds_swizzle(__double2loint(v), pattern);
+    const int hi = __builtin_amdgcn_ds_swizzle(__double2hiint(v), pattern);
+    return v + __hiloint2double(hi, lo);
+}
+
 // Sum over the kIcgnLanes lanes of a POI's group (a half-wavefront); every
 // lane of the group ends with the same bits.
 __device__ __forceinline__ double half_sum(double v)
 {
-#pragma unroll
-    for (int o = kIcgnLanes / 2; o > 0; o >>= 1) v += __shfl_xor(v, o);
-    return v;
+    static_assert(kIcgnLanes == 32 || kIcgnLanes == 16, "lane groups of 16 or 32");
+    if constexpr (kIcgnLanes == 32) v = add_lane_xor<16>(v);
+    v = add_lane_xor<8>(v);
+    v = add_lane_xor<4>(v);
+    v = add_lane_xor<2>(v);
+    return add_lane_xor<1>(v);
 }
 
 // True when pred holds in any active lane of the caller's group.
@@ -459,24 +473,21 @@ __device__ __forceinline__ bool subset_loop(const IcgnParams& P, int px, int py,
 // target pixels (:610-637); otherwise bicubic (:757-792).  Returns false when
 // a warped pixel leaves the ROI.  Accumulates sum T, sum T^2 and the six
 // sum SD_k T / 2 (the gradient plane stores 2 Rx, 2 Ry) in FP64.
-// The warp is affine, so over the subset its coordinates are extremal at the
-// four corners.  When every corner lands at least one pixel inside the
-// region where no per-pixel test can fire (the margin covers FP32 rounding
-// of the per-pixel warp, which is below 1e-3 px), the pass runs without the
-// per-pixel tests; the results are identical either way.
+// The warp is affine, so over the subset (offsets |dx| <= Sx, |dy| <= Sy) its
+// coordinates lie within POI + W2 -+ (|W0| Sx + |W1| Sy) (and likewise in
+// y).  When that range lies at least one pixel inside the region where no
+// per-pixel test can fire (the margin covers the FP32 rounding of the
+// per-pixel warp and of the range, both below 1e-2 px), the pass runs
+// without the per-pixel tests; the results are identical either way.
 template <int INTERP, bool INT, bool TABLE, bool RP, bool K32 = false>
 __device__ __forceinline__ bool subset_pass(const IcgnParams& P, int px, int py, const float Wm[6], int lane,
                                             double acc[8], const StepTable* tab)
 {
-    const float ex = (float)P.Sx, ey = (float)P.Sy, fx = (float)px, fy = (float)py;
-    const float x0 = warp_coord(fx, Wm[0], Wm[1], Wm[2], -ex, -ey), x1 = warp_coord(fx, Wm[0], Wm[1], Wm[2], ex, -ey);
-    const float x2 = warp_coord(fx, Wm[0], Wm[1], Wm[2], -ex, ey), x3 = warp_coord(fx, Wm[0], Wm[1], Wm[2], ex, ey);
-    const float y0 = warp_coord(fy, Wm[3], Wm[4], Wm[5], -ex, -ey), y1 = warp_coord(fy, Wm[3], Wm[4], Wm[5], ex, -ey);
-    const float y2 = warp_coord(fy, Wm[3], Wm[4], Wm[5], -ex, ey), y3 = warp_coord(fy, Wm[3], Wm[4], Wm[5], ex, ey);
-    const bool safe = fminf(fminf(x0, x1), fminf(x2, x3)) >= (float)(P.sx0 + 1) &&
-                      fmaxf(fmaxf(x0, x1), fmaxf(x2, x3)) < (float)(P.sx0 + P.rw - 2) &&
-                      fminf(fminf(y0, y1), fminf(y2, y3)) >= (float)(P.sy0 + 1) &&
-                      fmaxf(fmaxf(y0, y1), fmaxf(y2, y3)) < (float)(P.sy0 + P.rh - 2);
+    const float ex = (float)P.Sx, ey = (float)P.Sy;
+    const float cx = (float)px + Wm[2], cy = (float)py + Wm[5];
+    const float rx = fabsf(Wm[0] * ex) + fabsf(Wm[1] * ey), ry = fabsf(Wm[3] * ex) + fabsf(Wm[4] * ey);
+    const bool safe = cx - rx >= (float)(P.sx0 + 1) && cx + rx < (float)(P.sx0 + P.rw - 2) &&
+                      cy - ry >= (float)(P.sy0 + 1) && cy + ry < (float)(P.sy0 + P.rh - 2);
     // both halves take the same loop: the per-pixel tests run unless every
     // active lane's POI is safe
     if (__all(safe)) return subset_loop<INTERP, INT, false, TABLE, RP, K32>(P, px, py, Wm, lane, acc, tab);
