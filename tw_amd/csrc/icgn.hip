@@ -250,19 +250,36 @@ __device__ __forceinline__ void params_of(const float W[6], float P[6])
     P[5] = W[4] - 1;
 }
 
+// a / b correctly rounded from y = RN(1 / b) by Markstein's correction
+// (P. Markstein, IBM J. Res. Dev. 34(1), 1990: when y is within half an ulp
+// of 1/b and q0 within an ulp of a/b, the remainder r = a - b q0 is exact and
+// RN(q0 + r y) = RN(a / b)), two FMAs instead of a full division per
+// quotient.  A zero numerator returns q0 = a y, whose zero has the sign of
+// a / b (the correction would add a +0 to it).  Used only away from overflow
+// and subnormal quotients: |b| > FLT_EPSILON and the numerators are warp
+// terms of order 1 or exact zeros.
+__device__ __forceinline__ float div_rcp(float a, float b, float y)
+{
+    const float q0 = a * y;
+    const float r = fmaf(-q0, b, a);
+    return a == 0.f ? q0 : fmaf(r, y, q0);
+}
+
 // W <- W(p) o W(dp)^-1 with p = params_of(W): TW_paDIC_ICGN2D_CPU.cpp:713-745,
-// in FP32 with every operation rounded separately (no contraction).
+// in FP32 with every operation rounded separately (no contraction); the six
+// quotients by den are the correctly rounded ones, through one reciprocal.
 __device__ __forceinline__ void warp_update(const float d[6], float den, float W[6])
 {
 #pragma clang fp contract(off)
     float P[6];
     params_of(W, P);
-    float w00 = ((1 + P[1]) * (1 + d[5]) - P[2] * d[4]) / den;
-    float w01 = (P[2] * (1 + d[1]) - (1 + P[1]) * d[2]) / den;
-    float w02 = P[0] + (P[2] * (d[0] * d[4] - d[3] - d[3] * d[1]) - (1 + P[1]) * (d[0] * d[5] + d[0] - d[2] * d[3])) / den;
-    float w10 = (P[4] * (1 + d[5]) - (1 + P[5]) * d[4]) / den;
-    float w11 = ((1 + P[5]) * (1 + d[1]) - P[4] * d[2]) / den;
-    float w12 = P[3] + ((1 + P[5]) * (d[0] * d[4] - d[3] - d[3] * d[1]) - P[4] * (d[0] * d[5] + d[0] - d[2] * d[3])) / den;
+    const float y = 1.f / den;
+    float w00 = div_rcp((1 + P[1]) * (1 + d[5]) - P[2] * d[4], den, y);
+    float w01 = div_rcp(P[2] * (1 + d[1]) - (1 + P[1]) * d[2], den, y);
+    float w02 = P[0] + div_rcp(P[2] * (d[0] * d[4] - d[3] - d[3] * d[1]) - (1 + P[1]) * (d[0] * d[5] + d[0] - d[2] * d[3]), den, y);
+    float w10 = div_rcp(P[4] * (1 + d[5]) - (1 + P[5]) * d[4], den, y);
+    float w11 = div_rcp((1 + P[5]) * (1 + d[1]) - P[4] * d[2], den, y);
+    float w12 = P[3] + div_rcp((1 + P[5]) * (d[0] * d[4] - d[3] - d[3] * d[1]) - P[4] * (d[0] * d[5] + d[0] - d[2] * d[3]), den, y);
     W[0] = w00; W[1] = w01; W[2] = w02; W[3] = w10; W[4] = w11; W[5] = w12;
 }
 
