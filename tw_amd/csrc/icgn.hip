@@ -551,18 +551,18 @@ __device__ __forceinline__ void icgn_poi_t(const IcgnParams& P, int poi, int lan
                 flag = TWDIC_ICGN_SINGULAR_HESSIAN;
                 break;
             }
-            // RHS_k = sum SD_k ((R_s/T_s) (T - mean T) - (R - mean R))   (:657-667);
-            // R_s and mean R are read here, not held across the pass, which
-            // keeps the pass within the register budget of five waves per SIMD
+            // RHS_k = sum SD_k ((R_s/T_s) (T - mean T) - (R - mean R))   (:657-667),
+            // the last sum from the reference state; R_s is read here, not
+            // held across the pass, which keeps the pass within the register
+            // budget of five waves per SIMD
             int at = 0;
             asm volatile("" : "+v"(at));
-            const double normR = rs[RS_NORM + at], meanR = rs[RS_MEAN + at];
+            const double normR = rs[RS_NORM + at];
             const double nor = normR / normT;
             double r[6];
 #pragma unroll
             for (int k = 0; k < 6; ++k)
-                r[k] = __fma_rn(nor, centred(acc[2 + k], meanT, rs[RS_A + k]),
-                                -centred(rs[RS_B + k], meanR, rs[RS_A + k]));
+                r[k] = __fma_rn(nor, centred(acc[2 + k], meanT, rs[RS_A + k]), -rs[RS_B + k]);
             // L y = r, L^T x = y with the FP64 factor (spotrs 'L', :683, :853)
 #pragma unroll
             for (int i = 0; i < 6; ++i) {

@@ -402,7 +402,7 @@ __device__ __forceinline__ void ref_state_poi(const IcgnParams& P, int poi, int 
 #pragma unroll
         for (int i = 0; i < 6; ++i) {
             rs[RS_A + i] = a[i];
-            rs[RS_B + i] = br[i];
+            rs[RS_B + i] = centred(br[i], mean, a[i]);
         }
         rs[RS_NORM] = spread;
         rs[RS_FLAG] = flag;

@@ -19,7 +19,7 @@ constexpr int kRefStateStride = 48;   // doubles of per-reference state per POI 
 // Per-reference, per-POI state (doubles, kRefStateStride per POI):
 //   [0..20]  FP64 Cholesky factor L of the 6x6 Hessian, packed lower, row-major
 //   [21..26] A_k = sum SD_k               (steepest-descent sums)
-//   [27..32] BR_k = sum SD_k R            (raw reference projections)
+//   [27..32] B_k = sum SD_k (R - mean R)  (BR_k - mean R A_k with BR_k = sum SD_k R, one FMA)
 //   [33]     |R - mean R|_2               (R_s)
 //   [34]     reference flag: 0 ok, TWDIC_ICGN_DARK_SUBSET, TWDIC_ICGN_SINGULAR_HESSIAN,
 //            TWDIC_ICGN_OUT_OF_ROI (subset leaves the ROI)
