@@ -1,6 +1,6 @@
 """bench.py's contract on the CPU: asked for more than one GPU it exits with
-status 2 and says why, before touching a device (every GPU box here has one
-GPU, and the multi-GPU layout is a design note)."""
+status 2 and says why, before touching a device (the build targets one GPU;
+the multi-GPU layout is a design note)."""
 import subprocess
 import sys
 from pathlib import Path

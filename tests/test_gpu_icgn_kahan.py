@@ -1,4 +1,4 @@
-"""The IC-GN sums as north_star names them, FP32 with compensated (Kahan)
+"""The IC-GN sums in FP32 with compensated (Kahan)
 addition (TWDIC_ICGN_VARIANT=20), held to the same parity bars as the
 default FP64 sums on the 512^2 workload.  The variant exists to measure that
 alternative against the default (on gfx950 an FP64 FMA costs the same issue

@@ -5,7 +5,7 @@ host code (C ABI, batch and serving layers) compiled with both sanitizers,
 UBSan aborting on its first finding, and the GPU objects as in the library.
 Its driver (tw_amd/cpp/examples/host_check.cpp) runs seeded random
 configurations through validation and creation, the CSV writer and synthetic
-pairs; on a GPU box it also runs a context through every entry point, a batch
+pairs; with a GPU it also runs a context through every entry point, a batch
 in both launch forms against the context bit for bit, and a burst of
 requests through a server.  Leak detection is off: the HIP runtime keeps its
 allocations to the end of the process."""
@@ -32,7 +32,7 @@ def _run(timeout):
 
 
 def test_host_code_sanitized_without_device():
-    """Part A everywhere (on a GPU box part B runs too, under the gpu test)."""
+    """Part A everywhere (with a GPU, part B runs too, under the gpu test)."""
     out = _run(600)
     assert "host_check:" in out
 

@@ -304,8 +304,8 @@ __device__ __forceinline__ double conv_norm(const float d[6], int Sx, int Sy)
 }  // namespace
 
 // FP32 sums with compensated (Kahan) addition of each pixel's T, T^2 and
-// SD_k T / 2 terms, every product and sum rounded separately: the accumulation
-// north_star names, built as a measured alternative to the FP64 sums
+// SD_k T / 2 terms, every product and sum rounded separately: the compensated
+// FP32 accumulation, built as a measured alternative to the FP64 sums
 // (TWDIC_ICGN_VARIANT=20), not the default.  The FP32 products SD_k T round
 // here, where the FP64 ones are exact.
 __device__ __forceinline__ void kahan_pixel(float ks[8], float kc[8], float T, uint32_t g, float dx, float dy)
