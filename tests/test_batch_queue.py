@@ -7,8 +7,10 @@ The host builds, per image pair i with n_i POIs (batch_api.cpp set_pairs):
   * fitems[c]: the same per FFT-CC window class c = 0 (N = 64), 1 (N = 32),
     2 (other N) (FFT-CC, persistent form, two items per workgroup grab);
   * mitems[f]: per matrix-core kernel form (FFT-CC, flat form);
-  * wg[t]: (i, k) for k = 0, 8, 16, ... of each pair, t = 0 when the subset fits
-    the step table (IC-GN, flat form, eight POIs per workgroup).
+  * wg[t + 2 w]: (i, k) for k = 0, G, 2 G, ... of each pair, t = 0 when the
+    subset fits the step table, w = 0 with G = 8 POIs per workgroup (one per
+    half-wavefront) for pairs of at least kIcgnHalfWaveMin POIs, else w = 1
+    with G = 4 (IC-GN, flat form).
 and each persistent loop is
   ref state:  i = atomicAdd(c, 1)     per wavefront; exit when i >= total
   IC-GN:      b = atomicAdd(c, chunk) per wavefront; items [b, min(b + chunk, total));
@@ -31,10 +33,14 @@ from tests import cases
 
 CSRC = Path(__file__).resolve().parents[1] / "tw_amd" / "csrc"
 # POIs per IC-GN workgroup item, as the library defines it: one per group of
-# kIcgnLanes lanes of a 256-thread workgroup
+# icgn_lanes_for(n) lanes of a 256-thread workgroup
 _HDR = (CSRC / "twdic_internal.h").read_text()
-assert "constexpr int kIcgnPoisPerWg = 256 / kIcgnLanes;" in _HDR
-POIS_PER_WG = 256 // int(re.search(r"constexpr int kIcgnLanes = (\d+);", _HDR).group(1))
+assert "inline int icgn_lanes_for(int n) { return n >= kIcgnHalfWaveMin ? 32 : 64; }" in _HDR
+HALF_WAVE_MIN = int(re.search(r"constexpr int kIcgnHalfWaveMin = (\d+);", _HDR).group(1))
+
+
+def pois_per_wg(n):
+    return 256 // (32 if n >= HALF_WAVE_MIN else 64)
 
 
 def mma_form(N):
@@ -54,10 +60,10 @@ def work_lists(layout):
     items = np.stack([pair, poi], 1)
     fitems = [items[np.isin(pair, [i for i, N in enumerate(Ns) if fclass(N) == c])] for c in range(3)]
     mitems = [items[np.isin(pair, [i for i, N in enumerate(Ns) if mma_form(N) == f])] for f in range(4)]
-    wg = [[], []]
+    wg = [[], [], [], []]
     for i, n in enumerate(counts):
-        t = 0 if side[i] * side[i] <= 2048 and side[i] <= 48 else 1
-        wg[t].extend((i, k) for k in range(0, n, POIS_PER_WG))
+        t = (0 if side[i] * side[i] <= 2048 and side[i] <= 48 else 1) + (0 if pois_per_wg(n) == 8 else 2)
+        wg[t].extend((i, k) for k in range(0, n, pois_per_wg(n)))
     wg = [np.array(w, np.int64).reshape(-1, 2) for w in wg]
     return counts, items, fitems, mitems, wg
 
@@ -82,12 +88,13 @@ def test_lists_name_only_existing_pois(c3):
     assert counts.sum() == 10_553_540 and len(items) == counts.sum()
     for lst in [items, *fitems, *mitems]:
         check_entries(lst, counts)
-    for w in wg:
-        check_entries(w, counts, POIS_PER_WG)
+    for t, w in enumerate(wg):
+        check_entries(w, counts, 8 if t < 2 else 4)
     # every POI of every pair is in exactly one class / form list and in one
-    # eight-POI workgroup item
+    # workgroup item
     assert sum(len(f) for f in fitems) == len(items) and sum(len(m) for m in mitems) == len(items)
-    assert sum(len(w) for w in wg) == int(sum((n + POIS_PER_WG - 1) // POIS_PER_WG for n in counts))
+    assert sum(len(w) for w in wg) == int(sum((n + pois_per_wg(n) - 1) // pois_per_wg(n) for n in counts))
+    assert len(wg[0]) + len(wg[1]) > 0 and len(wg[2]) + len(wg[3]) > 0       # both widths occur in the mix
 
 
 def emulate(total, actors, grab, cover, rng, max_steps):

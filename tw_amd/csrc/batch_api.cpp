@@ -63,8 +63,11 @@ struct twdic_batch {
     bool flat = true;                 // TWDIC_BATCH_FORM=direct selects the direct form
     int2* d_mitems[4] = {};           // FFT-CC items (pair, POI) per matrix-core kernel form (mma_form)
     int64_t nm[4] = {};
-    int2* d_wg[2] = {};               // IC-GN workgroup items (pair, first POI): step table fits, does not
-    int64_t nwg[2] = {};
+    // IC-GN workgroup items (pair, first POI), list t + 2 w: step table fits
+    // (t = 0) or not (t = 1), half-wavefront (w = 0) or whole wavefront (w = 1)
+    // per POI, by the pair's POI count (icgn_lanes_for)
+    int2* d_wg[4] = {};
+    int64_t nwg[4] = {};
     // persistent form: grid sizes from each kernel's own occupancy, computed
     // when the pairs are set (never inside a stream capture)
     int pblocks_state = 0, pblocks_icgn = 0, pblocks_fftcc[3][5] = {};
@@ -108,11 +111,11 @@ void release(twdic_batch* b)
     b->graph = nullptr;
     void* bufs[] = {b->d_desc, b->d_items, b->d_fitems[0], b->d_fitems[1], b->d_fitems[2], b->d_g2, b->d_state,
                     b->d_zncc_scratch, b->d_counters, b->d_mitems[0], b->d_mitems[1], b->d_mitems[2],
-                    b->d_mitems[3], b->d_wg[0], b->d_wg[1], b->d_rp};
+                    b->d_mitems[3], b->d_wg[0], b->d_wg[1], b->d_wg[2], b->d_wg[3], b->d_rp};
     for (void* p : bufs)
         if (p) (void)hipFree(p);
     for (int f = 0; f < 4; ++f) { b->d_mitems[f] = nullptr; b->nm[f] = 0; }
-    for (int t = 0; t < 2; ++t) { b->d_wg[t] = nullptr; b->nwg[t] = 0; }
+    for (int t = 0; t < 4; ++t) { b->d_wg[t] = nullptr; b->nwg[t] = 0; }
     for (int c = 0; c < 3; ++c) b->nf[c] = 0;
     for (auto& kv : b->tw) (void)hipFree(kv.second);
     b->tw.clear();
@@ -268,10 +271,11 @@ twdic_status enqueue_flat(twdic_batch* b, bool fftcc, bool icgn)
             if (b->nm[f]) TB_HIP(b, launch_batch_fftcc_mma(b->d_desc, b->d_mitems[f], (int)b->nm[f], f, b->stream));
     if (icgn) TB_HIP(b, launch_batch_row_poly(b->d_desc, (int)b->pairs.size(), b->max_rh, b->stream));
     if (icgn)
-        for (int t = 0; t < 2; ++t)
+        for (int t = 0; t < 4; ++t)
             if (b->nwg[t])
-                TB_HIP(b, launch_batch_icgn_wg(b->d_desc, b->d_wg[t], (int)b->nwg[t], t == 0, b->tune.icgn_waves,
-                                               b->tune.icgn_xcd_ranges != 0, b->all_rp, b->stream));
+                TB_HIP(b, launch_batch_icgn_wg(b->d_desc, b->d_wg[t], (int)b->nwg[t], (t & 1) == 0, t < 2 ? 32 : 64,
+                                               b->tune.icgn_waves, b->tune.icgn_xcd_ranges != 0, b->all_rp,
+                                               b->stream));
     return TWDIC_OK;
 }
 
@@ -374,6 +378,7 @@ twdic_status set_pairs_impl(twdic_batch* b, const twdic_pair* pairs, int npairs)
         q.sx0 = p.roi_x; q.sy0 = p.roi_y; q.rw = p.roi_w; q.rh = p.roi_h;
         q.Sx = p.sx; q.Sy = p.sy; q.Wsub = 2 * p.sx + 1; q.Hsub = 2 * p.sy + 1; q.s = q.Wsub * q.Hsub;
         q.max_iter = p.max_iter; q.delta_p = p.delta_p;
+        q.lanes = icgn_lanes_for(p.n);    // as a single context on this pair's POIs
         q.g2 = b->d_g2 + g2_off; q.lut = nullptr;
         // the fused interpolation's row polynomials, as the single context
         // builds them (ROIs past 2^28 pixels read the image bytes there too)
@@ -401,14 +406,15 @@ twdic_status set_pairs_impl(twdic_batch* b, const twdic_pair* pairs, int npairs)
         }
     }
     if (total) TB_HIP(b, hipMemcpy(b->d_items, items.data(), sizeof(int2) * total, hipMemcpyHostToDevice));
-    // flat form: FFT-CC items per kernel form, IC-GN items of kIcgnPoisPerWg POIs of one pair
-    std::vector<int2> mitems[4], wg[2];
+    // flat form: FFT-CC items per kernel form, IC-GN items of 256 / lanes POIs of one pair
+    std::vector<int2> mitems[4], wg[4];
     for (int i = 0; i < npairs; ++i) {
         const twdic_pair& p = b->pairs[i];
         auto& mi = mitems[mma_form(p.fft_n)];
         for (int k = 0; k < p.n; ++k) mi.push_back(make_int2(i, k));
-        auto& w = wg[step_table_fits(2 * p.sx + 1, 2 * p.sy + 1) ? 0 : 1];
-        for (int k = 0; k < p.n; k += kIcgnPoisPerWg) w.push_back(make_int2(i, k));
+        const int lanes = icgn_lanes_for(p.n), per_wg = 256 / lanes;
+        auto& w = wg[(step_table_fits(2 * p.sx + 1, 2 * p.sy + 1) ? 0 : 1) + (lanes == 64 ? 2 : 0)];
+        for (int k = 0; k < p.n; k += per_wg) w.push_back(make_int2(i, k));
     }
     for (int f = 0; f < 4; ++f) {
         b->nm[f] = (int64_t)mitems[f].size();
@@ -417,7 +423,7 @@ twdic_status set_pairs_impl(twdic_batch* b, const twdic_pair* pairs, int npairs)
         TB_HIP(b, alloc((void**)&b->d_mitems[f], sizeof(int2) * mitems[f].size()));
         TB_HIP(b, hipMemcpy(b->d_mitems[f], mitems[f].data(), sizeof(int2) * mitems[f].size(), hipMemcpyHostToDevice));
     }
-    for (int t = 0; t < 2; ++t) {
+    for (int t = 0; t < 4; ++t) {
         b->nwg[t] = (int64_t)wg[t].size();
         if (!b->nwg[t]) continue;
         TB_HIP(b, alloc((void**)&b->d_wg[t], sizeof(int2) * wg[t].size()));

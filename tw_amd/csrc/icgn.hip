@@ -1,5 +1,7 @@
-// icgn.hip -- first-order IC-GN sub-pixel registration for gfx950, one
-// 32-lane half of a wavefront per POI, eight POIs per 256-thread workgroup.
+// icgn.hip -- first-order IC-GN sub-pixel registration for gfx950: one POI
+// per 32-lane half of a wavefront, eight per 256-thread workgroup, for large
+// POI sets; one POI per wavefront, four per workgroup, for small ones
+// (icgn_lanes_for).
 //
 // Semantics follow ICGN2D_CPU::ICGN2D_Compute (TW_paDIC_ICGN2D_CPU.cpp:
 // 481-925): first pass at integer target pixels, later passes bicubic; flags
@@ -9,21 +11,23 @@
 // FP32 dp with the norm evaluated in FP64 as the reference's pow() does.
 //
 // Per-iteration work stays in registers:
-//   * each lane of a half walks pixels l, l+32, ... (l = lane & 31) of the
-//     (2Sy+1)x(2Sx+1) subset;
+//   * each lane of a POI's group of L lanes walks pixels l, l+L, ...
+//     (l = lane % L) of the (2Sy+1)x(2Sx+1) subset;
 //   * one pass forms sum T, sum T^2 and the six sums SD_k T in FP64 (every
 //     FP32 product SD_k T is exact in FP64), so the residual vector is
 //       RHS_k = (R_s/T_s) (sum SD_k T - mean(T) A_k) - B_k
 //     with A_k = sum SD_k and B_k = sum SD_k (R - mean R) precomputed per
 //     reference (k_ref_state).  This equals the reference's three passes
 //     (:793-826) algebraically and is more accurate;
-//   * butterflies within the half reduce the 8 sums (its 32 lanes end
+//   * butterflies within the group reduce the 8 sums (its lanes end
 //     bit-identical);
 //   * the 6x6 solve uses the per-POI FP64 Cholesky factor.
-// The per-POI work between passes (reductions, solve, warp update, loop
-// test) is the same instruction stream for the two POIs of a wavefront, so
-// it is issued once for both; a half whose POI has finished idles until the
-// other's has.
+// With half-wavefronts the per-POI work between passes (reductions, solve,
+// warp update, loop test) is one instruction stream for the two POIs of a
+// wavefront, issued once for both; a half whose POI has finished idles until
+// the other's has.  That halves the cost of that work when the POIs fill the
+// GPU many times over; with few POIs a lone wavefront's latency per pass
+// binds, and a whole wavefront per POI takes half the pixel rounds.
 #include "twdic_internal.h"
 
 #include <cfloat>
@@ -43,12 +47,14 @@ __device__ __forceinline__ double add_lane_xor(double v)
     return v + __hiloint2double(hi, lo);
 }
 
-// Sum over the kIcgnLanes lanes of a POI's group (a half-wavefront); every
+// Sum over the L lanes of a POI's group (a wavefront or a half); every
 // lane of the group ends with the same bits.
+template <int L>
 __device__ __forceinline__ double half_sum(double v)
 {
-    static_assert(kIcgnLanes == 32 || kIcgnLanes == 16, "lane groups of 16 or 32");
-    if constexpr (kIcgnLanes == 32) v = add_lane_xor<16>(v);
+    static_assert(L == 64 || L == 32, "lane groups of 64 or 32");
+    if constexpr (L == 64) v += __shfl_xor(v, 32);
+    v = add_lane_xor<16>(v);
     v = add_lane_xor<8>(v);
     v = add_lane_xor<4>(v);
     v = add_lane_xor<2>(v);
@@ -56,10 +62,12 @@ __device__ __forceinline__ double half_sum(double v)
 }
 
 // True when pred holds in any active lane of the caller's group.
+template <int L>
 __device__ __forceinline__ bool half_any(bool pred, int lane)
 {
-    const unsigned long long b = __ballot(pred) >> (lane & (64 - kIcgnLanes));
-    return (b & ((1ull << kIcgnLanes) - 1)) != 0ull;
+    const unsigned long long b = __ballot(pred);
+    if constexpr (L == 64) return b != 0ull;
+    return (uint32_t)((lane & 32) ? (b >> 32) : b) != 0u;
 }
 
 // Warp coordinate exactly as the CPU path evaluates it (:618-619, :761-762):
@@ -348,22 +356,22 @@ __device__ __forceinline__ void fill_step_table(StepTable& t, const IcgnParams& 
     }
 }
 
-// One pass over the subset at the current warp, by the 32 lanes of a half
-// (hl = lane & 31; px, py and the warp are the half's POI's).  INT: first pass
-// at integer target pixels (:610-637); otherwise bicubic (:757-792).  Returns
-// false when a warped pixel of the half's POI leaves the ROI.  Accumulates
-// sum T, sum T^2 and the six sum SD_k T / 2 (the gradient plane stores 2 Rx,
-// 2 Ry) in FP64.  With a step table (tab != null) the lanes take pixels
-// hl + 32 k from it: s / 32 full rounds, then the s % 32 remaining pixels;
+// One pass over the subset at the current warp, by the L lanes of a POI's
+// group (hl = lane % L; px, py and the warp are the group's POI's).  INT:
+// first pass at integer target pixels (:610-637); otherwise bicubic
+// (:757-792).  Returns false when a warped pixel of the group's POI leaves the
+// ROI.  Accumulates sum T, sum T^2 and the six sum SD_k T / 2 (the gradient
+// plane stores 2 Rx, 2 Ry) in FP64.  With a step table (tab != null) the lanes
+// take pixels hl + L k from it: s / L full rounds, then the s % L remaining pixels;
 // without one they step their offsets in registers.  Each lane visits the
 // same pixels in the same order either way, so the sums are identical.  RP:
 // the fused bicubic passes evaluate the target's row polynomials (P.rp)
 // instead of the image bytes.
-template <int INTERP, bool INT, bool CHECK, bool TABLE, bool RP, bool K32 = false>
+template <int L, int INTERP, bool INT, bool CHECK, bool TABLE, bool RP, bool K32 = false>
 __device__ __forceinline__ bool subset_loop(const IcgnParams& P, int px, int py, const float Wm[6], int lane,
                                             double acc[8], const StepTable* tab)
 {
-    const int hl = lane & (kIcgnLanes - 1);
+    const int hl = lane & (L - 1);
     // the fused interpolation from the bytes returns T 2^-24 (interp_rows_s);
     // the sums are scaled back at the end, exactly
     constexpr bool SCALED = !INT && INTERP == TWDIC_INTERP_FUSED && !RP;
@@ -436,22 +444,22 @@ __device__ __forceinline__ bool subset_loop(const IcgnParams& P, int px, int py,
     // byte offset of the subset's corner pixel in the gradient plane
     const int gcorner = 4 * ((py - Sy - sy0) * rw + (px - Sx - sx0));
     if constexpr (TABLE) {
-        const int full = s / kIcgnLanes;
+        const int full = s / L;
 #pragma unroll 1
         for (int k = 0; k < full; ++k) {
-            const int p = hl + kIcgnLanes * k;
+            const int p = hl + L * k;
             pixel(tab->dx[p], tab->dy[p], tab->go[p], gcorner);
         }
-        const int p = hl + kIcgnLanes * full;
+        const int p = hl + L * full;
         if (p < s) {
             pixel(tab->dx[p], tab->dy[p], tab->go[p], gcorner);
         }
     } else {
-        // lane walks pixels p = hl, hl + L, ... (L = kIcgnLanes): column
+        // lane walks pixels p = hl, hl + L, ...: column
         // m = p % Wsub, row l = p / Wsub, kept as the exact float offsets
         // dx = m - Sx, dy = l - Sy and the 32-bit byte offset into the
         // gradient plane
-        const int qL = kIcgnLanes / Wsub, rL = kIcgnLanes - qL * Wsub;
+        const int qL = L / Wsub, rL = L - qL * Wsub;
         // recomputed per pass: kept live across the iteration loop, this start
         // state is what the register allocator spills
         int lane_v = hl;
@@ -462,7 +470,7 @@ __device__ __forceinline__ bool subset_loop(const IcgnParams& P, int px, int py,
         const float fSx = (float)Sx, fWsub = (float)Wsub, frL = (float)rL, fqL = (float)qL;
         const uint32_t gstep = 4u * (uint32_t)(qL * rw + rL), gwrap = 4u * (uint32_t)(rw - Wsub);
 #pragma unroll 1
-        for (int p = hl; p < s; p += kIcgnLanes) {
+        for (int p = hl; p < s; p += L) {
             pixel(dx, dy, go, 0);
             // next pixel of this lane: L further in row-major subset order
             dx += frL;
@@ -483,7 +491,7 @@ __device__ __forceinline__ bool subset_loop(const IcgnParams& P, int px, int py,
     acc[0] = sc * st; acc[1] = sc2 * stt;
     acc[2] = hsc * a0; acc[3] = hsc * a1; acc[4] = hsc * a2;
     acc[5] = hsc * b0; acc[6] = hsc * b1; acc[7] = hsc * b2;
-    return CHECK ? !half_any(out, lane) : true;
+    return CHECK ? !half_any<L>(out, lane) : true;
 }
 
 // One pass over the subset at the current warp.  INT: first pass at integer
@@ -496,7 +504,7 @@ __device__ __forceinline__ bool subset_loop(const IcgnParams& P, int px, int py,
 // per-pixel test can fire (the margin covers the FP32 rounding of the
 // per-pixel warp and of the range, both below 1e-2 px), the pass runs
 // without the per-pixel tests; the results are identical either way.
-template <int INTERP, bool INT, bool TABLE, bool RP, bool K32 = false>
+template <int L, int INTERP, bool INT, bool TABLE, bool RP, bool K32 = false>
 __device__ __forceinline__ bool subset_pass(const IcgnParams& P, int px, int py, const float Wm[6], int lane,
                                             double acc[8], const StepTable* tab)
 {
@@ -507,21 +515,27 @@ __device__ __forceinline__ bool subset_pass(const IcgnParams& P, int px, int py,
                       cy - ry >= (float)(P.sy0 + 1) && cy + ry < (float)(P.sy0 + P.rh - 2);
     // both halves take the same loop: the per-pixel tests run unless every
     // active lane's POI is safe
-    if (__all(safe)) return subset_loop<INTERP, INT, false, TABLE, RP, K32>(P, px, py, Wm, lane, acc, tab);
-    return subset_loop<INTERP, INT, true, TABLE, RP, K32>(P, px, py, Wm, lane, acc, tab);
+    if (__all(safe)) return subset_loop<L, INTERP, INT, false, TABLE, RP, K32>(P, px, py, Wm, lane, acc, tab);
+    return subset_loop<L, INTERP, INT, true, TABLE, RP, K32>(P, px, py, Wm, lane, acc, tab);
 }
 
-// IC-GN for one POI by one half of a wavefront (poi is the half's; the other
-// half may run another POI of the same launch or none).  rs: the POI's
-// reference state, copied to LDS by the caller (its first kRefStateUsed
-// doubles).  Every lane of the half holds the same per-POI values, so the two
-// halves' solves and warp updates issue as one instruction stream.
-template <int INTERP, bool TABLE, bool RP, bool K32 = false>
+// IC-GN for one POI by a group of L lanes: a whole wavefront (L = 64) or
+// one half (L = 32; the other half may run another POI of the same launch or
+// none).  rs: the POI's reference state, copied to LDS by the caller (its
+// first kRefStateUsed doubles).  Every lane of the group holds the same
+// per-POI values, so with halves the two POIs' solves and warp updates issue
+// as one instruction stream; with whole wavefronts they are wave-uniform and
+// kept in SGPRs.
+template <int L, int INTERP, bool TABLE, bool RP, bool K32 = false>
 __device__ __forceinline__ void icgn_poi_t(const IcgnParams& P, int poi, int lane, const StepTable* tab,
                                            const double* rs)
 {
-    const int py = P.poi[2 * poi];
-    const int px = P.poi[2 * poi + 1];
+    int py = P.poi[2 * poi];
+    int px = P.poi[2 * poi + 1];
+    if constexpr (L == 64) {
+        py = __builtin_amdgcn_readfirstlane(py);
+        px = __builtin_amdgcn_readfirstlane(px);
+    }
     const int Sx = P.Sx, Sy = P.Sy;
     const double dn = (double)P.s;
 
@@ -538,11 +552,11 @@ __device__ __forceinline__ void icgn_poi_t(const IcgnParams& P, int poi, int lan
         for (int it = 0;; ++it) {
             if (it >= 1) iters = it + 1;                       // ++iNumIterations (:753)
             double acc[8];
-            const bool inside = it == 0 ? subset_pass<INTERP, true, TABLE, RP, K32>(P, px, py, Wm, lane, acc, tab)
-                                        : subset_pass<INTERP, false, TABLE, RP, K32>(P, px, py, Wm, lane, acc, tab);
+            const bool inside = it == 0 ? subset_pass<L, INTERP, true, TABLE, RP, K32>(P, px, py, Wm, lane, acc, tab)
+                                        : subset_pass<L, INTERP, false, TABLE, RP, K32>(P, px, py, Wm, lane, acc, tab);
             if (!inside) { flag = TWDIC_ICGN_OUT_OF_ROI; break; }         // :632-635, :787-790
 #pragma unroll
-            for (int i = 0; i < 8; ++i) acc[i] = half_sum(acc[i]);
+            for (int i = 0; i < 8; ++i) acc[i] = half_sum<L>(acc[i]);
             double meanT, normT;
             mean_spread(acc[0], acc[1], dn, meanT, normT);
             if (fabsf((float)meanT) <= FLT_EPSILON) { flag = TWDIC_ICGN_DARK_SUBSET; break; }   // :638-640
@@ -584,13 +598,19 @@ __device__ __forceinline__ void icgn_poi_t(const IcgnParams& P, int poi, int lan
             const float den = warp_den(d);
             if (fabsf(den) <= FLT_EPSILON) { flag = TWDIC_ICGN_SINGULAR_WARP; break; }        // :875-877
             warp_update(d, den, Wm);
+            if constexpr (L == 64) {
+                // the butterflies leave the sums, hence the warp, bit-identical
+                // in all lanes: keep it in SGPRs for the next pass
+#pragma unroll
+                for (int i = 0; i < 6; ++i) Wm[i] = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(Wm[i])));
+            }
             const double nrm = conv_norm(d, Sx, Sy);
             dpn = (float)nrm;
             if (it == 0) iters = 1;                                    // :749
             if (!(iters < P.max_iter && nrm >= P.delta_p)) break;
         }
     }
-    if ((lane & (kIcgnLanes - 1)) == 0) {
+    if ((lane & (L - 1)) == 0) {
         float Pp[6];
         params_of(Wm, Pp);
         P.u[poi] = Pp[0];
@@ -608,14 +628,14 @@ __device__ __forceinline__ void icgn_poi_t(const IcgnParams& P, int poi, int lan
 // The fused mode reads the target's row polynomials when the caller built
 // them (P.rp, the same for every wave of a launch), else the image bytes;
 // both evaluate the same Catmull-Rom spline.
-template <int INTERP, bool TABLE>
+template <int L, int INTERP, bool TABLE>
 __device__ __forceinline__ void icgn_poi(const IcgnParams& P, int poi, int lane, const StepTable* tab,
                                          const double* rs)
 {
     if (INTERP == TWDIC_INTERP_FUSED && P.rp != nullptr)
-        icgn_poi_t<INTERP, TABLE, INTERP == TWDIC_INTERP_FUSED>(P, poi, lane, tab, rs);
+        icgn_poi_t<L, INTERP, TABLE, INTERP == TWDIC_INTERP_FUSED>(P, poi, lane, tab, rs);
     else
-        icgn_poi_t<INTERP, TABLE, false>(P, poi, lane, tab, rs);
+        icgn_poi_t<L, INTERP, TABLE, false>(P, poi, lane, tab, rs);
 }
 
 // The reference state of the POIs first .. first + count - 1 (those below n)
@@ -636,9 +656,9 @@ __device__ __forceinline__ void copy_ref_state(double (*slots)[kRefStateUsed], c
 // 0 leaves the choice to the compiler.
 // TABLE: the subset has at most kStepTable pixels; the workgroup builds the
 // step table in LDS first.
-// Half a wavefront per POI, eight per workgroup (POI 8 b' + t / 32 for thread
-// t); consecutive workgroups go to the 8 XCDs round-robin, so block b is
-// remapped to b' = the next block of XCD b % 8's contiguous share and each
+// L lanes per POI, 256 / L POIs per workgroup (POI (256 / L) b' + t / L for
+// thread t); consecutive workgroups go to the 8 XCDs round-robin, so block b
+// is remapped to b' = the next block of XCD b % 8's contiguous share and each
 // XCD's L2 holds one band of the image.
 __device__ __forceinline__ int xcd_block(int b, int nb)
 {
@@ -646,60 +666,63 @@ __device__ __forceinline__ int xcd_block(int b, int nb)
     return x * q + min(x, r) + slot;
 }
 
-template <int INTERP, int MINW, bool TABLE, bool RP, bool K32 = false>
+template <int INTERP, int MINW, bool TABLE, bool RP, int L, bool K32 = false>
 __global__ __launch_bounds__(256, MINW > 0 ? MINW : 1) void k_icgn(IcgnParams P)
 {
-    __shared__ double rs[kIcgnPoisPerWg][kRefStateUsed];
-    const int first = xcd_block(blockIdx.x, gridDim.x) * kIcgnPoisPerWg;
-    const int slot = threadIdx.x / kIcgnLanes, poi = first + slot;
-    copy_ref_state(rs, P.refstate, first, kIcgnPoisPerWg, P.n, threadIdx.x, 256);
+    constexpr int G = 256 / L;                   // POIs per workgroup
+    __shared__ double rs[G][kRefStateUsed];
+    const int first = xcd_block(blockIdx.x, gridDim.x) * G;
+    const int slot = threadIdx.x / L, poi = first + slot;
+    copy_ref_state(rs, P.refstate, first, G, P.n, threadIdx.x, 256);
     if constexpr (TABLE) {
         __shared__ StepTable tab;
         fill_step_table(tab, P);
         __syncthreads();
         if (poi >= P.n) return;
-        icgn_poi_t<INTERP, true, RP, K32>(P, poi, threadIdx.x & 63, &tab, rs[slot]);
+        icgn_poi_t<L, INTERP, true, RP, K32>(P, poi, threadIdx.x & 63, &tab, rs[slot]);
     } else {
         __syncthreads();
         if (poi >= P.n) return;
-        icgn_poi_t<INTERP, false, RP, K32>(P, poi, threadIdx.x & 63, nullptr, rs[slot]);
+        icgn_poi_t<L, INTERP, false, RP, K32>(P, poi, threadIdx.x & 63, nullptr, rs[slot]);
     }
 }
 
 // Batched form, one launch over many image pairs: workgroup item w =
-// (pair, first POI) holds up to eight POIs of one pair, so the workgroup's
-// step table is that pair's.  XCD: items in contiguous per-XCD ranges as in
+// (pair, first POI) holds up to 256 / L POIs of one pair (every listed pair
+// takes L lanes per POI, icgn_lanes_for of its POI count), so the
+// workgroup's step table is that pair's.  XCD: items in contiguous per-XCD ranges as in
 // k_icgn (one band of the work per L2); otherwise dealt out in block order,
 // which balances the XCDs when the work per item differs from pair to pair.
 // ALL_RP: every pair has its row-polynomial plane (the usual case), so only
 // that form is compiled in and gets the register budget; otherwise each pair
 // takes its form at run time.
-template <int INTERP, int MINW, bool TABLE, bool XCD, bool ALL_RP>
+template <int INTERP, int MINW, bool TABLE, bool XCD, bool ALL_RP, int L>
 __global__ __launch_bounds__(256, MINW > 0 ? MINW : 1) void k_batch_icgn_wg(const PairDesc* __restrict__ pairs,
                                                                            const int2* __restrict__ wg)
 {
+    constexpr int G = 256 / L;
     const int2 it = wg[XCD ? xcd_block(blockIdx.x, gridDim.x) : (int)blockIdx.x];
     const IcgnParams P = const_pairs(pairs)[__builtin_amdgcn_readfirstlane(it.x)].icgn;
-    __shared__ double rs[kIcgnPoisPerWg][kRefStateUsed];
+    __shared__ double rs[G][kRefStateUsed];
     const int first = __builtin_amdgcn_readfirstlane(it.y);
-    const int slot = threadIdx.x / kIcgnLanes, poi = first + slot;
-    copy_ref_state(rs, P.refstate, first, kIcgnPoisPerWg, P.n, threadIdx.x, 256);
+    const int slot = threadIdx.x / L, poi = first + slot;
+    copy_ref_state(rs, P.refstate, first, G, P.n, threadIdx.x, 256);
     if constexpr (TABLE) {
         __shared__ StepTable tab;
         fill_step_table(tab, P);
         __syncthreads();
         if (poi >= P.n) return;
         if constexpr (ALL_RP)
-            icgn_poi_t<INTERP, true, true>(P, poi, threadIdx.x & 63, &tab, rs[slot]);
+            icgn_poi_t<L, INTERP, true, true>(P, poi, threadIdx.x & 63, &tab, rs[slot]);
         else
-            icgn_poi<INTERP, true>(P, poi, threadIdx.x & 63, &tab, rs[slot]);
+            icgn_poi<L, INTERP, true>(P, poi, threadIdx.x & 63, &tab, rs[slot]);
     } else {
         __syncthreads();
         if (poi >= P.n) return;
         if constexpr (ALL_RP)
-            icgn_poi_t<INTERP, false, true>(P, poi, threadIdx.x & 63, nullptr, rs[slot]);
+            icgn_poi_t<L, INTERP, false, true>(P, poi, threadIdx.x & 63, nullptr, rs[slot]);
         else
-            icgn_poi<INTERP, false>(P, poi, threadIdx.x & 63, nullptr, rs[slot]);
+            icgn_poi<L, INTERP, false>(P, poi, threadIdx.x & 63, nullptr, rs[slot]);
     }
 }
 
@@ -708,34 +731,44 @@ bool step_table_fits(int wsub, int hsub)
     return wsub * hsub <= kStepTable;
 }
 
-template <int MINW, bool TABLE, bool XCD, bool ALL_RP>
+template <int MINW, bool TABLE, bool XCD, bool ALL_RP, int L>
 static void launch_wg(const PairDesc* pairs, const int2* wg, int count, hipStream_t st)
 {
-    hipLaunchKernelGGL((k_batch_icgn_wg<TWDIC_INTERP_FUSED, MINW, TABLE, XCD, ALL_RP>), dim3(count), dim3(256), 0, st,
-                       pairs, wg);
+    hipLaunchKernelGGL((k_batch_icgn_wg<TWDIC_INTERP_FUSED, MINW, TABLE, XCD, ALL_RP, L>), dim3(count), dim3(256), 0,
+                       st, pairs, wg);
 }
 
-template <int MINW, bool ALL_RP>
+template <int MINW, bool ALL_RP, int L>
 static void launch_wg_w(const PairDesc* pairs, const int2* wg, int count, bool table, bool xcd, hipStream_t st)
 {
     if (table)
-        xcd ? launch_wg<MINW, true, true, ALL_RP>(pairs, wg, count, st)
-            : launch_wg<MINW, true, false, ALL_RP>(pairs, wg, count, st);
+        xcd ? launch_wg<MINW, true, true, ALL_RP, L>(pairs, wg, count, st)
+            : launch_wg<MINW, true, false, ALL_RP, L>(pairs, wg, count, st);
     else
-        xcd ? launch_wg<MINW, false, true, ALL_RP>(pairs, wg, count, st)
-            : launch_wg<MINW, false, false, ALL_RP>(pairs, wg, count, st);
+        xcd ? launch_wg<MINW, false, true, ALL_RP, L>(pairs, wg, count, st)
+            : launch_wg<MINW, false, false, ALL_RP, L>(pairs, wg, count, st);
 }
 
-hipError_t launch_batch_icgn_wg(const PairDesc* pairs, const int2* wg, int count, bool table, int waves,
+template <int L>
+static void launch_wg_l(const PairDesc* pairs, const int2* wg, int count, bool table, int waves, bool xcd_ranges,
+                        bool all_rp, hipStream_t st)
+{
+    if (all_rp)
+        waves == 6 ? launch_wg_w<6, true, L>(pairs, wg, count, table, xcd_ranges, st)
+                   : launch_wg_w<5, true, L>(pairs, wg, count, table, xcd_ranges, st);
+    else
+        waves == 6 ? launch_wg_w<6, false, L>(pairs, wg, count, table, xcd_ranges, st)
+                   : launch_wg_w<5, false, L>(pairs, wg, count, table, xcd_ranges, st);
+}
+
+hipError_t launch_batch_icgn_wg(const PairDesc* pairs, const int2* wg, int count, bool table, int lanes, int waves,
                                 bool xcd_ranges, bool all_rp, hipStream_t st)
 {
     if (count <= 0) return hipSuccess;
-    if (all_rp)
-        waves == 6 ? launch_wg_w<6, true>(pairs, wg, count, table, xcd_ranges, st)
-                   : launch_wg_w<5, true>(pairs, wg, count, table, xcd_ranges, st);
+    if (lanes == 64)
+        launch_wg_l<64>(pairs, wg, count, table, waves, xcd_ranges, all_rp, st);
     else
-        waves == 6 ? launch_wg_w<6, false>(pairs, wg, count, table, xcd_ranges, st)
-                   : launch_wg_w<5, false>(pairs, wg, count, table, xcd_ranges, st);
+        launch_wg_l<32>(pairs, wg, count, table, waves, xcd_ranges, all_rp, st);
     return hipGetLastError();
 }
 
@@ -743,7 +776,8 @@ hipError_t launch_batch_icgn_wg(const PairDesc* pairs, const int2* wg, int count
 // (image pair, POI) from a device counter until the list is exhausted; each
 // item's geometry and buffers come from its pair's descriptor.  Every wave
 // exits once the counter passes `total`.  Consecutive items may belong to
-// different pairs, so each item runs on the wavefront's lower half alone.
+// different pairs, so each item runs alone, on its pair's L lanes (the whole
+// wavefront, or its lower half).
 __global__ __launch_bounds__(256) void k_batch_icgn(const PairDesc* __restrict__ pairs, const int2* __restrict__ items,
                                                     int total, int* counter, int chunk)
 {
@@ -764,7 +798,10 @@ __global__ __launch_bounds__(256) void k_batch_icgn(const PairDesc* __restrict__
             copy_ref_state(rs[wave], P.refstate, poi, 1, P.n, lane, 64);
             __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
             __builtin_amdgcn_wave_barrier();
-            if (lane < kIcgnLanes) icgn_poi<TWDIC_INTERP_FUSED, false>(P, poi, lane, nullptr, rs[wave][0]);
+            if (P.lanes == 64)
+                icgn_poi<64, TWDIC_INTERP_FUSED, false>(P, poi, lane, nullptr, rs[wave][0]);
+            else if (lane < 32)
+                icgn_poi<32, TWDIC_INTERP_FUSED, false>(P, poi, lane, nullptr, rs[wave][0]);
         }
     }
 }
@@ -785,26 +822,38 @@ hipError_t launch_batch_icgn(const PairDesc* pairs, const int2* items, int total
     return hipGetLastError();
 }
 
-template <int INTERP, bool TABLE, bool RP>
+// variant: the occupancy target of the half-wavefront form (5, 6, 7 waves per
+// SIMD or 0, the compiler's choice), or 20, the compensated FP32 sums; the
+// whole-wavefront form is built for 5 waves per SIMD.
+template <int INTERP, bool TABLE, bool RP, int L>
 static void launch_icgn_v(const IcgnParams& p, int variant, hipStream_t st)
 {
-    const dim3 grid((p.n + kIcgnPoisPerWg - 1) / kIcgnPoisPerWg);
-    switch (variant) {
-    case 20: hipLaunchKernelGGL((k_icgn<INTERP, 5, TABLE, RP, true>), grid, dim3(256), 0, st, p); break;
-    case 5: hipLaunchKernelGGL((k_icgn<INTERP, 5, TABLE, RP>), grid, dim3(256), 0, st, p); break;
-    case 6: hipLaunchKernelGGL((k_icgn<INTERP, 6, TABLE, RP>), grid, dim3(256), 0, st, p); break;
-    case 7: hipLaunchKernelGGL((k_icgn<INTERP, 7, TABLE, RP>), grid, dim3(256), 0, st, p); break;
-    default: hipLaunchKernelGGL((k_icgn<INTERP, 0, TABLE, RP>), grid, dim3(256), 0, st, p); break;
+    constexpr int G = 256 / L;
+    const dim3 grid((p.n + G - 1) / G);
+    if (variant == 20) {
+        hipLaunchKernelGGL((k_icgn<INTERP, 5, TABLE, RP, L, true>), grid, dim3(256), 0, st, p);
+        return;
+    }
+    if constexpr (L == 64) {
+        hipLaunchKernelGGL((k_icgn<INTERP, 5, TABLE, RP, 64>), grid, dim3(256), 0, st, p);
+    } else {
+        switch (variant) {
+        case 5: hipLaunchKernelGGL((k_icgn<INTERP, 5, TABLE, RP, L>), grid, dim3(256), 0, st, p); break;
+        case 6: hipLaunchKernelGGL((k_icgn<INTERP, 6, TABLE, RP, L>), grid, dim3(256), 0, st, p); break;
+        case 7: hipLaunchKernelGGL((k_icgn<INTERP, 7, TABLE, RP, L>), grid, dim3(256), 0, st, p); break;
+        default: hipLaunchKernelGGL((k_icgn<INTERP, 0, TABLE, RP, L>), grid, dim3(256), 0, st, p); break;
+        }
     }
 }
 
 template <int INTERP, bool RP>
 static void launch_icgn_r(const IcgnParams& p, int variant, hipStream_t st)
 {
-    if (p.s <= kStepTable)
-        launch_icgn_v<INTERP, true, RP>(p, variant, st);
+    const bool table = p.s <= kStepTable;
+    if (p.lanes == 64)
+        table ? launch_icgn_v<INTERP, true, RP, 64>(p, variant, st) : launch_icgn_v<INTERP, false, RP, 64>(p, variant, st);
     else
-        launch_icgn_v<INTERP, false, RP>(p, variant, st);
+        table ? launch_icgn_v<INTERP, true, RP, 32>(p, variant, st) : launch_icgn_v<INTERP, false, RP, 32>(p, variant, st);
 }
 
 template <int INTERP>
@@ -819,6 +868,7 @@ static void launch_icgn_t(const IcgnParams& p, int variant, hipStream_t st)
 hipError_t launch_icgn(const IcgnParams& p, int interp, int variant, hipStream_t st)
 {
     if (p.n <= 0) return hipSuccess;
+    if (p.lanes != 32 && p.lanes != 64) return hipErrorInvalidValue;
     if (interp == TWDIC_INTERP_FUSED)
         launch_icgn_t<TWDIC_INTERP_FUSED>(p, variant, st);
     else if (interp == TWDIC_INTERP_BSPLINE)

@@ -362,7 +362,7 @@ bool admit(twdic_server* s, Slot& sl, Job& j)
     ip.Sx = ip.Sy = r.subset; ip.Wsub = ip.Hsub = 2 * r.subset + 1; ip.s = ip.Wsub * ip.Hsub;
     ip.max_iter = r.max_iter; ip.delta_p = r.delta_p;
     ip.g2 = d_g2; ip.lut = nullptr; ip.rp = d_rp; ip.refstate = d_state;
-    ip.poi = d_poi; ip.n = n;
+    ip.poi = d_poi; ip.n = n; ip.lanes = icgn_lanes_for(n);
     ip.u = d_u; ip.v = d_v; ip.p6 = d_p6; ip.iters = d_it; ip.flags = d_fl; ip.dpnorm = nullptr;
     if (build) {
         chk(launch_ref_state(ip, sl.st));

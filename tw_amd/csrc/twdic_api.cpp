@@ -192,6 +192,7 @@ IcgnParams icgn_params(twdic_ctx* c, const int32_t* poi, int n, float* u, float*
     p.rp = g.interp == TWDIC_INTERP_FUSED ? c->d_rp : nullptr;    // null when the plane does not fit
     p.refstate = c->d_refstate;
     p.poi = poi; p.n = n;
+    p.lanes = icgn_lanes_for(n);
     p.u = u; p.v = v; p.p6 = p6; p.iters = iters; p.flags = flags; p.dpnorm = dpnorm;
     return p;
 }

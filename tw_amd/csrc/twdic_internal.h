@@ -28,11 +28,15 @@ constexpr int kRefStateStride = 48;   // doubles of per-reference state per POI 
 enum { RS_L = 0, RS_A = 21, RS_B = 27, RS_NORM = 33, RS_FLAG = 34, RS_MEAN = 35, RS_RDIAG = 36 };
 constexpr int kRefStateUsed = 42;     // the doubles above that the iteration kernels read
 
-// IC-GN runs one POI per group of kIcgnLanes lanes of a wavefront (a half),
-// eight POIs per 256-thread workgroup; the batched form's workgroup items hold
-// this many POIs of a pair.
-constexpr int kIcgnLanes = 32;
-constexpr int kIcgnPoisPerWg = 256 / kIcgnLanes;
+// IC-GN runs one POI per group of L lanes of a wavefront, 256 / L POIs per
+// 256-thread workgroup: L = 32 (a half-wavefront) for sets of at least
+// kIcgnHalfWaveMin POIs, where the two POIs of a wavefront share the per-pass
+// work between the pixel loops; L = 64 for smaller sets, which leave the GPU
+// partly idle and run faster with half the pixel rounds per pass.  The
+// choice follows the whole set's POI count (also for a chunk of it, and per
+// pair in a batch), so a POI's result does not depend on the path that ran it.
+constexpr int kIcgnHalfWaveMin = 32768;
+inline int icgn_lanes_for(int n) { return n >= kIcgnHalfWaveMin ? 32 : 64; }
 
 // Mean and L2 spread of n samples from their exact-or-FP64 sums s1 = sum x,
 // s2 = sum x^2.  Shared by the reference-state and iteration kernels so that
@@ -73,6 +77,7 @@ struct IcgnParams {
     int sx0, sy0, rw, rh;     // ROI
     int Sx, Sy, Wsub, Hsub, s;
     int max_iter;
+    int lanes;                // lanes per POI: 32 or 64 (icgn_lanes_for of the set's POI count)
     double delta_p;           // the caller's float, widened: the loop test compares in FP64
     const uint32_t* g2;       // reference gradients (2Rx, 2Ry) as int16 pairs, ROI-relative
     const float* lut;         // target LUT [rh][rw][16]
@@ -140,7 +145,8 @@ hipError_t launch_fftcc(const FftccParams& p, int variant, hipStream_t st);
 int mma_form(int N);
 hipError_t launch_batch_fftcc_mma(const PairDesc* pairs, const int2* items, int count, int form, hipStream_t st);
 // batched form: IC-GN (fused interpolation) over workgroup items (pair, first
-// POI), kIcgnPoisPerWg POIs of one pair per workgroup; `table` when every listed pair's
+// POI), 256 / lanes POIs of one pair per workgroup, every listed pair with
+// that many lanes per POI; `table` when every listed pair's
 // subset fits the step table (step_table_fits).
 bool step_table_fits(int wsub, int hsub);
 // The row-polynomial plane of an ROI is addressed with 32-bit byte offsets
@@ -156,7 +162,7 @@ inline size_t row_poly_bytes(int rw, int rh)
 }
 // waves: occupancy target 5 or 6; xcd_ranges: contiguous per-XCD item ranges;
 // all_rp: every pair's descriptor has its row-polynomial plane
-hipError_t launch_batch_icgn_wg(const PairDesc* pairs, const int2* wg, int count, bool table, int waves,
+hipError_t launch_batch_icgn_wg(const PairDesc* pairs, const int2* wg, int count, bool table, int lanes, int waves,
                                 bool xcd_ranges, bool all_rp, hipStream_t st);
 hipError_t launch_ref_state(const IcgnParams& p, hipStream_t st);
 hipError_t launch_icgn(const IcgnParams& p, int interp, int variant, hipStream_t st);
