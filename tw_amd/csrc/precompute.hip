@@ -310,30 +310,49 @@ hipError_t launch_batch_row_poly(const PairDesc* pairs, int npairs, int max_rh, 
 }
 
 // ---------------------------------------------------------------------------
-// Per-reference state, one wavefront per POI (built once per reference and
-// POI set; the reference rebuilds it on m_isRefImgUpdated,
+// Per-reference state, one POI per group of L lanes (a wavefront, or a
+// half-wavefront for large POI sets, as the iterations; built once per
+// reference and POI set; the reference rebuilds it on m_isRefImgUpdated,
 // TW_paDIC_cuICGN2D.cu:883-903).  Sums follow ICGN2D_Compute's precompute
 // (TW_paDIC_ICGN2D_CPU.cpp:516-595): SD = [Rx, Rx dx, Rx dy, Ry, Ry dx, Ry dy],
 // H = sum SD^T SD.  Every SD is a half-integer multiple and every product is
 // a multiple of 1/4 below 2^24, so the FP64 sums here are exact and
-// independent of order.  The Cholesky factorisation is FP64.
+// independent of order: the group width does not change a bit of it.  The
+// Cholesky factorisation is FP64.
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ double wave_sum_d(double v)
+template <int O>
+__device__ __forceinline__ double add_lane_xor_d(double v)
 {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
-    return v;
+    // lane ^ O within 32-lane groups: ds_swizzle bit mode (and-mask 0x1f, or-mask 0, xor-mask O)
+    constexpr int pattern = 0x1f | (O << 10);
+    const int lo = __builtin_amdgcn_ds_swizzle(__double2loint(v), pattern);
+    const int hi = __builtin_amdgcn_ds_swizzle(__double2hiint(v), pattern);
+    return v + __hiloint2double(hi, lo);
 }
 
+template <int L>
+__device__ __forceinline__ double group_sum_d(double v)
+{
+    static_assert(L == 64 || L == 32, "lane groups of 64 or 32");
+    if constexpr (L == 64) v += __shfl_xor(v, 32);
+    v = add_lane_xor_d<16>(v);
+    v = add_lane_xor_d<8>(v);
+    v = add_lane_xor_d<4>(v);
+    v = add_lane_xor_d<2>(v);
+    return add_lane_xor_d<1>(v);
+}
+
+template <int L>
 __device__ __forceinline__ void ref_state_poi(const IcgnParams& P, int poi, int lane)
 {
+    const int hl = lane & (L - 1);
     const int py = P.poi[2 * poi], px = P.poi[2 * poi + 1];
     double* rs = P.refstate + (size_t)poi * kRefStateStride;
     // subsets that leave the ROI are flagged by the iteration kernel; keep the
     // state well defined and touch no memory outside the ROI
     if (px - P.Sx < P.sx0 || py - P.Sy < P.sy0 || px + P.Sx >= P.sx0 + P.rw ||
         py + P.Sy >= P.sy0 + P.rh) {
-        if (lane < kRefStateStride) rs[lane] = (lane == RS_FLAG) ? (double)TWDIC_ICGN_OUT_OF_ROI : 0.0;
+        for (int i = hl; i < kRefStateStride; i += L) rs[i] = (i == RS_FLAG) ? (double)TWDIC_ICGN_OUT_OF_ROI : 0.0;
         return;
     }
     double h[21], a[6], br[6];
@@ -343,9 +362,9 @@ __device__ __forceinline__ void ref_state_poi(const IcgnParams& P, int poi, int 
     for (int i = 0; i < 6; ++i) { a[i] = 0.0; br[i] = 0.0; }
     long long sr = 0, srr = 0;
     const int W = P.Wsub, s = P.s;
-    const int q64 = kWave / W, r64 = kWave - q64 * W;
-    int l = lane / W, m = lane - l * W;
-    for (int p = lane; p < s; p += kWave) {
+    const int q64 = L / W, r64 = L - q64 * W;
+    int l = hl / W, m = hl - l * W;
+    for (int p = hl; p < s; p += L) {
         int gy = py - P.Sy + l, gxp = px - P.Sx + m;
         int R = P.ref[(size_t)gy * P.stride + gxp];
         const uint32_t g = P.g2[(size_t)(gy - P.sy0) * P.rw + (gxp - P.sx0)];
@@ -365,10 +384,10 @@ __device__ __forceinline__ void ref_state_poi(const IcgnParams& P, int poi, int 
         if (m >= W) { m -= W; l += 1; }
     }
 #pragma unroll
-    for (int i = 0; i < 21; ++i) h[i] = wave_sum_d(h[i]);
+    for (int i = 0; i < 21; ++i) h[i] = group_sum_d<L>(h[i]);
 #pragma unroll
-    for (int i = 0; i < 6; ++i) { a[i] = wave_sum_d(a[i]); br[i] = wave_sum_d(br[i]); }
-    double dsr = wave_sum_d((double)sr), dsrr = wave_sum_d((double)srr);  // exact (< 2^53)
+    for (int i = 0; i < 6; ++i) { a[i] = group_sum_d<L>(a[i]); br[i] = group_sum_d<L>(br[i]); }
+    double dsr = group_sum_d<L>((double)sr), dsrr = group_sum_d<L>((double)srr);  // exact (< 2^53)
 
     double mean, spread;
     mean_spread(dsr, dsrr, (double)s, mean, spread);
@@ -376,29 +395,29 @@ __device__ __forceinline__ void ref_state_poi(const IcgnParams& P, int poi, int 
     // TW_paDIC_ICGN2D_CPU.cpp:580-595: zero mean or zero spread -> DarkSubset.
     // Both sums are exact integers, so the test is exact: all-zero or uniform.
     if (dsr == 0.0 || dsrr * (double)s == dsr * dsr) flag = (double)TWDIC_ICGN_DARK_SUBSET;
-    double L[21];
+    double Lf[21];
 #pragma unroll
-    for (int i = 0; i < 21; ++i) L[i] = h[i];
+    for (int i = 0; i < 21; ++i) Lf[i] = h[i];
     if (flag == 0.0) {
         // Cholesky (spotrf 'L', TW_paDIC_ICGN2D_CPU.cpp:682,688-693) in FP64
         for (int j = 0; j < 6; ++j) {
             int jj = j * (j + 1) / 2;
-            double d = L[jj + j];
-            for (int k = 0; k < j; ++k) d -= L[jj + k] * L[jj + k];
+            double d = Lf[jj + j];
+            for (int k = 0; k < j; ++k) d -= Lf[jj + k] * Lf[jj + k];
             if (!(d > 0.0)) { flag = (double)TWDIC_ICGN_SINGULAR_HESSIAN; break; }
             d = sqrt(d);
-            L[jj + j] = d;
+            Lf[jj + j] = d;
             for (int i = j + 1; i < 6; ++i) {
                 int ii = i * (i + 1) / 2;
-                double v = L[ii + j];
-                for (int k = 0; k < j; ++k) v -= L[ii + k] * L[jj + k];
-                L[ii + j] = v / d;
+                double v = Lf[ii + j];
+                for (int k = 0; k < j; ++k) v -= Lf[ii + k] * Lf[jj + k];
+                Lf[ii + j] = v / d;
             }
         }
     }
-    if (lane == 0) {
+    if (hl == 0) {
 #pragma unroll
-        for (int i = 0; i < 21; ++i) rs[RS_L + i] = L[i];
+        for (int i = 0; i < 21; ++i) rs[RS_L + i] = Lf[i];
 #pragma unroll
         for (int i = 0; i < 6; ++i) {
             rs[RS_A + i] = a[i];
@@ -408,20 +427,26 @@ __device__ __forceinline__ void ref_state_poi(const IcgnParams& P, int poi, int 
         rs[RS_FLAG] = flag;
         rs[RS_MEAN] = mean;
 #pragma unroll
-        for (int i = 0; i < 6; ++i) rs[RS_RDIAG + i] = flag == 0.0 ? 1.0 / L[i * (i + 1) / 2 + i] : 0.0;
+        for (int i = 0; i < 6; ++i) rs[RS_RDIAG + i] = flag == 0.0 ? 1.0 / Lf[i * (i + 1) / 2 + i] : 0.0;
     }
 }
 
+template <int L>
 __global__ __launch_bounds__(256) void k_ref_state(IcgnParams P)
 {
-    const int poi = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int poi = blockIdx.x * (256 / L) + (int)(threadIdx.x / L);
     if (poi >= P.n) return;
-    ref_state_poi(P, poi, threadIdx.x & 63);
+    ref_state_poi<L>(P, poi, threadIdx.x & 63);
 }
 
+// The POI set's lane width (p.lanes): large sets two POIs per wavefront, so
+// the 35 group reductions are issued once for both.
 hipError_t launch_ref_state(const IcgnParams& p, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_ref_state, dim3((p.n + 3) / 4), dim3(256), 0, st, p);
+    if (p.lanes == 32)
+        hipLaunchKernelGGL(k_ref_state<32>, dim3((p.n + 7) / 8), dim3(256), 0, st, p);
+    else
+        hipLaunchKernelGGL(k_ref_state<64>, dim3((p.n + 3) / 4), dim3(256), 0, st, p);
     return hipGetLastError();
 }
 
@@ -437,7 +462,7 @@ __global__ __launch_bounds__(256) void k_batch_ref_state(const PairDesc* __restr
         if (i >= total) break;
         const int2 it = items[i];
         const IcgnParams P = const_pairs(pairs)[__builtin_amdgcn_readfirstlane(it.x)].icgn;
-        ref_state_poi(P, __builtin_amdgcn_readfirstlane(it.y), lane);
+        ref_state_poi<64>(P, __builtin_amdgcn_readfirstlane(it.y), lane);
     }
 }
 
