@@ -50,7 +50,7 @@ __device__ __forceinline__ double add_lane_xor(double v)
 // Sum over the L lanes of a POI's group (a wavefront or a half); every
 // lane of the group ends with the same bits.
 template <int L>
-__device__ __forceinline__ double half_sum(double v)
+__device__ __forceinline__ double group_sum(double v)
 {
     static_assert(L == 64 || L == 32, "lane groups of 64 or 32");
     if constexpr (L == 64) v += __shfl_xor(v, 32);
@@ -63,7 +63,7 @@ __device__ __forceinline__ double half_sum(double v)
 
 // True when pred holds in any active lane of the caller's group.
 template <int L>
-__device__ __forceinline__ bool half_any(bool pred, int lane)
+__device__ __forceinline__ bool group_any(bool pred, int lane)
 {
     const unsigned long long b = __ballot(pred);
     if constexpr (L == 64) return b != 0ull;
@@ -491,7 +491,7 @@ __device__ __forceinline__ bool subset_loop(const IcgnParams& P, int px, int py,
     acc[0] = sc * st; acc[1] = sc2 * stt;
     acc[2] = hsc * a0; acc[3] = hsc * a1; acc[4] = hsc * a2;
     acc[5] = hsc * b0; acc[6] = hsc * b1; acc[7] = hsc * b2;
-    return CHECK ? !half_any<L>(out, lane) : true;
+    return CHECK ? !group_any<L>(out, lane) : true;
 }
 
 // One pass over the subset at the current warp.  INT: first pass at integer
@@ -556,7 +556,7 @@ __device__ __forceinline__ void icgn_poi_t(const IcgnParams& P, int poi, int lan
                                         : subset_pass<L, INTERP, false, TABLE, RP, K32>(P, px, py, Wm, lane, acc, tab);
             if (!inside) { flag = TWDIC_ICGN_OUT_OF_ROI; break; }         // :632-635, :787-790
 #pragma unroll
-            for (int i = 0; i < 8; ++i) acc[i] = half_sum<L>(acc[i]);
+            for (int i = 0; i < 8; ++i) acc[i] = group_sum<L>(acc[i]);
             double meanT, normT;
             mean_spread(acc[0], acc[1], dn, meanT, normT);
             if (fabsf((float)meanT) <= FLT_EPSILON) { flag = TWDIC_ICGN_DARK_SUBSET; break; }   // :638-640
