@@ -439,11 +439,13 @@ __global__ __launch_bounds__(256) void k_ref_state(IcgnParams P)
     ref_state_poi<L>(P, poi, threadIdx.x & 63);
 }
 
-// The POI set's lane width (p.lanes): large sets two POIs per wavefront, so
-// the 35 group reductions are issued once for both.
+// Two POIs per wavefront from 4,096 POIs on, so the 35 group reductions are
+// issued once for both (their own crossover: the sums are exact, so the
+// width need not follow the iterations'); smaller sets one per wavefront,
+// half the pixel rounds for a GPU left partly idle.
 hipError_t launch_ref_state(const IcgnParams& p, hipStream_t st)
 {
-    if (p.lanes == 32)
+    if (p.n >= 4096)
         hipLaunchKernelGGL(k_ref_state<32>, dim3((p.n + 7) / 8), dim3(256), 0, st, p);
     else
         hipLaunchKernelGGL(k_ref_state<64>, dim3((p.n + 3) / 4), dim3(256), 0, st, p);
