@@ -822,9 +822,10 @@ hipError_t launch_batch_icgn(const PairDesc* pairs, const int2* items, int total
     return hipGetLastError();
 }
 
-// variant: the occupancy target of the half-wavefront form (5, 6, 7 waves per
-// SIMD or 0, the compiler's choice), or 20, the compensated FP32 sums; the
-// whole-wavefront form is built for 5 waves per SIMD.
+// variant: the occupancy target of the half-wavefront form (5 or 6 waves per
+// SIMD, or 0, the compiler's choice), or 20, the compensated FP32 sums; the
+// whole-wavefront form is built for 5 waves per SIMD.  (Seven waves do not
+// fit: the workgroup's LDS caps a CU at five.)
 template <int INTERP, bool TABLE, bool RP, int L>
 static void launch_icgn_v(const IcgnParams& p, int variant, hipStream_t st)
 {
@@ -840,7 +841,6 @@ static void launch_icgn_v(const IcgnParams& p, int variant, hipStream_t st)
         switch (variant) {
         case 5: hipLaunchKernelGGL((k_icgn<INTERP, 5, TABLE, RP, L>), grid, dim3(256), 0, st, p); break;
         case 6: hipLaunchKernelGGL((k_icgn<INTERP, 6, TABLE, RP, L>), grid, dim3(256), 0, st, p); break;
-        case 7: hipLaunchKernelGGL((k_icgn<INTERP, 7, TABLE, RP, L>), grid, dim3(256), 0, st, p); break;
         default: hipLaunchKernelGGL((k_icgn<INTERP, 0, TABLE, RP, L>), grid, dim3(256), 0, st, p); break;
         }
     }
