@@ -445,7 +445,8 @@ __device__ __forceinline__ bool subset_loop(const IcgnParams& P, int px, int py,
     const int gcorner = 4 * ((py - Sy - sy0) * rw + (px - Sx - sx0));
     if constexpr (TABLE) {
         const int full = s / L;
-#pragma unroll 1
+        // two pixels per trip: their loads are in flight together
+#pragma unroll 2
         for (int k = 0; k < full; ++k) {
             const int p = hl + L * k;
             pixel(tab->dx[p], tab->dy[p], tab->go[p], gcorner);
