@@ -366,8 +366,8 @@ __device__ __forceinline__ void fill_step_table(StepTable& t, const IcgnParams& 
 // without one they step their offsets in registers.  Each lane visits the
 // same pixels in the same order either way, so the sums are identical.  RP:
 // the fused bicubic passes evaluate the target's row polynomials (P.rp)
-// instead of the image bytes.
-template <int L, int INTERP, bool INT, bool CHECK, bool TABLE, bool RP, bool K32 = false>
+// instead of the image bytes.  UNR: pixels per trip of the step-table loop.
+template <int L, int INTERP, bool INT, bool CHECK, bool TABLE, bool RP, bool K32 = false, int UNR = 1>
 __device__ __forceinline__ bool subset_loop(const IcgnParams& P, int px, int py, const float Wm[6], int lane,
                                             double acc[8], const StepTable* tab)
 {
@@ -445,8 +445,8 @@ __device__ __forceinline__ bool subset_loop(const IcgnParams& P, int px, int py,
     const int gcorner = 4 * ((py - Sy - sy0) * rw + (px - Sx - sx0));
     if constexpr (TABLE) {
         const int full = s / L;
-        // two pixels per trip: their loads are in flight together
-#pragma unroll 2
+        // with UNR = 2 two pixels' loads are in flight together
+#pragma unroll UNR
         for (int k = 0; k < full; ++k) {
             const int p = hl + L * k;
             pixel(tab->dx[p], tab->dy[p], tab->go[p], gcorner);
@@ -505,7 +505,7 @@ __device__ __forceinline__ bool subset_loop(const IcgnParams& P, int px, int py,
 // per-pixel test can fire (the margin covers the FP32 rounding of the
 // per-pixel warp and of the range, both below 1e-2 px), the pass runs
 // without the per-pixel tests; the results are identical either way.
-template <int L, int INTERP, bool INT, bool TABLE, bool RP, bool K32 = false>
+template <int L, int INTERP, bool INT, bool TABLE, bool RP, bool K32 = false, int UNR = 1>
 __device__ __forceinline__ bool subset_pass(const IcgnParams& P, int px, int py, const float Wm[6], int lane,
                                             double acc[8], const StepTable* tab)
 {
@@ -516,8 +516,8 @@ __device__ __forceinline__ bool subset_pass(const IcgnParams& P, int px, int py,
                       cy - ry >= (float)(P.sy0 + 1) && cy + ry < (float)(P.sy0 + P.rh - 2);
     // both halves take the same loop: the per-pixel tests run unless every
     // active lane's POI is safe
-    if (__all(safe)) return subset_loop<L, INTERP, INT, false, TABLE, RP, K32>(P, px, py, Wm, lane, acc, tab);
-    return subset_loop<L, INTERP, INT, true, TABLE, RP, K32>(P, px, py, Wm, lane, acc, tab);
+    if (__all(safe)) return subset_loop<L, INTERP, INT, false, TABLE, RP, K32, UNR>(P, px, py, Wm, lane, acc, tab);
+    return subset_loop<L, INTERP, INT, true, TABLE, RP, K32, UNR>(P, px, py, Wm, lane, acc, tab);
 }
 
 // IC-GN for one POI by a group of L lanes: a whole wavefront (L = 64) or
@@ -527,7 +527,7 @@ __device__ __forceinline__ bool subset_pass(const IcgnParams& P, int px, int py,
 // per-POI values, so with halves the two POIs' solves and warp updates issue
 // as one instruction stream; with whole wavefronts they are wave-uniform and
 // kept in SGPRs.
-template <int L, int INTERP, bool TABLE, bool RP, bool K32 = false>
+template <int L, int INTERP, bool TABLE, bool RP, bool K32 = false, int UNR = 1>
 __device__ __forceinline__ void icgn_poi_t(const IcgnParams& P, int poi, int lane, const StepTable* tab,
                                            const double* rs)
 {
@@ -553,8 +553,8 @@ __device__ __forceinline__ void icgn_poi_t(const IcgnParams& P, int poi, int lan
         for (int it = 0;; ++it) {
             if (it >= 1) iters = it + 1;                       // ++iNumIterations (:753)
             double acc[8];
-            const bool inside = it == 0 ? subset_pass<L, INTERP, true, TABLE, RP, K32>(P, px, py, Wm, lane, acc, tab)
-                                        : subset_pass<L, INTERP, false, TABLE, RP, K32>(P, px, py, Wm, lane, acc, tab);
+            const bool inside = it == 0 ? subset_pass<L, INTERP, true, TABLE, RP, K32, UNR>(P, px, py, Wm, lane, acc, tab)
+                                        : subset_pass<L, INTERP, false, TABLE, RP, K32, UNR>(P, px, py, Wm, lane, acc, tab);
             if (!inside) { flag = TWDIC_ICGN_OUT_OF_ROI; break; }         // :632-635, :787-790
 #pragma unroll
             for (int i = 0; i < 8; ++i) acc[i] = group_sum<L>(acc[i]);
@@ -680,7 +680,9 @@ __global__ __launch_bounds__(256, MINW > 0 ? MINW : 1) void k_icgn(IcgnParams P)
         fill_step_table(tab, P);
         __syncthreads();
         if (poi >= P.n) return;
-        icgn_poi_t<L, INTERP, true, RP, K32>(P, poi, threadIdx.x & 63, &tab, rs[slot]);
+        // two pixels per table-loop trip with halves (0.6 % faster on the
+        // 2048^2 frame); the batched kernel keeps one, which was 5 % faster there
+        icgn_poi_t<L, INTERP, true, RP, K32, L == 32 ? 2 : 1>(P, poi, threadIdx.x & 63, &tab, rs[slot]);
     } else {
         __syncthreads();
         if (poi >= P.n) return;
